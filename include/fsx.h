@@ -1,0 +1,113 @@
+/*
+ * fsx.h -- extended, re-entrant C ABI of the MI355X scoring path.
+ *
+ * No counterpart in the reference: it is the flat-array form of
+ * sxs_compute_saxs_scores (src/fftsaxs.h:95-105) that the drop-in wrapper
+ * calls, generalised to
+ *   - any beta grid size (beta_i = i*pi/(nbeta-1); the reference fixes L+1),
+ *   - any alpha_B and gamma_A/gamma_B grid sizes (reference: 2L+1),
+ *   - 64-bit pose indices (the reference's int index overflows, H7),
+ *   - caller-chosen c1/c2 bounds (reference: define.h:28-31),
+ *   - inputs and outputs in host or device (HBM) memory,
+ * and it returns error codes instead of exiting.
+ *
+ * Pose index: mixed radix [z][b1][b2][a2][g1][g2] with radices
+ * (znum, nbeta, nbeta, nalpha, ngamma, ngamma), exactly the reference
+ * encoding when nbeta = L+1 and nalpha = ngamma = 2L+1 (index.c:3-29).
+ *
+ * SPF arrays: [3][qnum][(L+1)^2] doubles, block 0 = V (vacuum), 1 = D
+ * (excluded volume), 2 = W (water shell); element l(l+1)+m holds A_lm
+ * (pdb2spf.h:41-53 flattened).
+ *
+ * Threading: calls may come from several host threads; calls on one device
+ * are serialised internally.
+ */
+#ifndef FSX_H
+#define FSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSX_OK 0
+#define FSX_ERR_ARG (-1)     /* invalid argument (sizes, NULL pointers) */
+#define FSX_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define FSX_ERR_NOMEM (-3)   /* host or device allocation failed */
+#define FSX_ERR_NODEV (-4)   /* no HIP device visible */
+
+/* flags for fsx_score */
+#define FSX_DEVICE_POSES 1u    /* poses points to device memory */
+#define FSX_DEVICE_OUTPUT 2u   /* score/c1/c2/cross point to device memory */
+#define FSX_ALL_POSES_VALID 4u /* caller guarantees every index is in range */
+
+/* pi of the reference CMake build (-std=c11: define.h:13-15, H1) */
+#define FSX_PI_REFERENCE 3.14159265358
+
+typedef struct fsx_problem {
+    int L;               /* expansion order, 1..40 */
+    int nbeta;           /* beta grid points on [0, pi] (reference: L+1) */
+    int nalpha;          /* alpha_B grid points on [0, 2pi) (reference: 2L+1) */
+    int ngamma;          /* gamma_A / gamma_B grid points (reference: 2L+1) */
+    int qnum;
+    const double *qvals;         /* host, qnum */
+    int znum;
+    const double *zvals;         /* host, znum (Angstrom) */
+    const double *A_re, *A_im;   /* receptor SPF, host [3][qnum][(L+1)^2] */
+    const double *B_re, *B_im;   /* ligand SPF, host */
+    const double *a;             /* host [6*qnum], scoring_helper moments */
+    double mult;                 /* (4pi/3)^1.5 rm^2 / (16pi), min_saxs.c:121 */
+    double peak;                 /* experimental I(q0), min_saxs.c:122 */
+    double c1_lo, c1_hi, c2_lo, c2_hi;   /* reference: 0.96, 1.04, -2, 4 */
+    double pi;                   /* pi of the beta grid (FSX_PI_REFERENCE) */
+} fsx_problem;
+
+typedef struct fsx_stats {
+    int n_scored;        /* poses with an in-range z slot */
+    int n_capped;        /* fits that hit the evaluation cap (never in parity runs) */
+    long fit_evals;      /* total objective evaluations */
+    double ms_tables;    /* host tables + uploads */
+    double ms_bessel_t;  /* Bessel series + translation matrices (GPU) */
+    double ms_rotate;    /* rotated receptor / ligand coefficients (GPU) */
+    double ms_group;     /* pose bucketing by (z, beta2, gamma2) (GPU) */
+    double ms_cross;     /* per-pose cross-term profiles (GPU) */
+    double ms_fit;       /* per-pose chi^2 fits (GPU) */
+    double ms_total;     /* whole call, wall clock */
+    double cross_bytes;  /* algorithmic bytes moved by the cross-term kernel */
+    double cross_flops;  /* executed flops of the cross-term kernel */
+} fsx_stats;
+
+/* Score n poses.  score/c1/c2 receive one value per pose whose z slot is
+ * in range; other entries are not written.  cross (optional, may be NULL)
+ * receives the un-rescaled cross-term profiles [n][6][qnum] (VV, VD, VW,
+ * DD, DW, WW; the values the reference fits, fftsaxs.c:877-905).  stats
+ * may be NULL.  Returns FSX_OK or a negative error code. */
+int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
+              double *c1, double *c2, double *cross, unsigned flags, fsx_stats *stats);
+
+/* Cross-term profile of SPF coefficients and fit of one profile on the GPU
+ * path's fit code compiled for the host (min_saxs.c:196-259 semantics).
+ * ct: [6][qnum] (already rescaled); out: score, c1, c2, evaluations. */
+int fsx_fit_host(const double *ct, const double *qvals, int qnum, const double *a,
+                 double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                 double *out4);
+
+/* Fit many profiles on the GPU: ct [n][6][qnum] (host), already rescaled.
+ * out [n][4]: score, c1, c2, evaluations. */
+int fsx_fit_batch(const double *ct, int n, const double *qvals, int qnum, const double *a,
+                  double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                  double *out);
+
+/* Bessel table j_p(x_i) for p = 0..pmax on the GPU, x host [n]; out [n][pmax+1] */
+int fsx_sbessel_batch(const double *x, int n, int pmax, double *out);
+
+int fsx_device_count(void);
+const char *fsx_error_string(int code);
+/* Release the cached device buffers of the calling process. */
+void fsx_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

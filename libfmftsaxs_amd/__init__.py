@@ -1,0 +1,249 @@
+"""Python binding (ctypes) of libfmftsaxs_amd, the MI355X implementation of
+libfmftsaxs's FFT dimer SAXS scoring path.
+
+The library is a C ABI (include/*.h); this module mirrors it for Python
+callers and tests.  Importing it loads the in-tree libfmftsaxs_amd.so and
+fails loudly if it has not been built (python -m libfmftsaxs_amd.build).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfmftsaxs_amd.so")
+PI_REFERENCE = 3.14159265358
+
+FSX_DEVICE_POSES = 1
+FSX_DEVICE_OUTPUT = 2
+
+
+class FsxError(RuntimeError):
+    pass
+
+
+class Problem(C.Structure):
+    _fields_ = [("L", C.c_int), ("nbeta", C.c_int), ("nalpha", C.c_int), ("ngamma", C.c_int),
+                ("qnum", C.c_int), ("qvals", C.POINTER(C.c_double)),
+                ("znum", C.c_int), ("zvals", C.POINTER(C.c_double)),
+                ("A_re", C.POINTER(C.c_double)), ("A_im", C.POINTER(C.c_double)),
+                ("B_re", C.POINTER(C.c_double)), ("B_im", C.POINTER(C.c_double)),
+                ("a", C.POINTER(C.c_double)), ("mult", C.c_double), ("peak", C.c_double),
+                ("c1_lo", C.c_double), ("c1_hi", C.c_double), ("c2_lo", C.c_double),
+                ("c2_hi", C.c_double), ("pi", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_scored", C.c_int), ("n_capped", C.c_int), ("fit_evals", C.c_long),
+                ("ms_tables", C.c_double), ("ms_bessel_t", C.c_double),
+                ("ms_rotate", C.c_double), ("ms_group", C.c_double),
+                ("ms_cross", C.c_double), ("ms_fit", C.c_double), ("ms_total", C.c_double),
+                ("cross_bytes", C.c_double), ("cross_flops", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SpfSing(C.Structure):
+    _fields_ = [("L", C.c_int), ("re", C.POINTER(C.c_double)), ("im", C.POINTER(C.c_double))]
+
+
+class SpfFull(C.Structure):
+    _fields_ = [("L", C.c_int), ("qnum", C.c_int), ("rm", C.c_double),
+                ("V", C.POINTER(C.POINTER(SpfSing))), ("D", C.POINTER(C.POINTER(SpfSing))),
+                ("W", C.POINTER(C.POINTER(SpfSing)))]
+
+
+class Profile(C.Structure):
+    _fields_ = [("qnum", C.c_int), ("rerr", C.c_double)] + [
+        (n, C.POINTER(C.c_double)) for n in ("in_", "err", "qvals", "VV", "VD", "VW", "DD",
+                                               "DW", "WW")] + [
+        (n, C.c_double) for n in ("score", "scale", "c1", "c2", "b1", "g1", "a2", "b2", "g2")]
+
+
+class OptParams(C.Structure):
+    _fields_ = [("rm", C.c_double), ("mult", C.c_double), ("peak", C.c_double),
+                ("a", C.POINTER(C.c_double)), ("wa", C.c_double * 43251),
+                ("iwa", C.c_long * 3072), ("dsave", C.c_double * 29),
+                ("isave", C.c_long * 44), ("lsave", C.c_long * 4)]
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FsxError("libfmftsaxs_amd.so is not built: run python -m libfmftsaxs_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.fsx_score.argtypes = [C.POINTER(Problem), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_uint, C.POINTER(Stats)]
+        L.fsx_score.restype = C.c_int
+        L.fsx_fit_host.argtypes = [_dp, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
+        L.fsx_fit_host.restype = C.c_int
+        L.fsx_fit_batch.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
+        L.fsx_fit_batch.restype = C.c_int
+        L.fsx_sbessel_batch.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.fsx_sbessel_batch.restype = C.c_int
+        L.fsx_device_count.restype = C.c_int
+        L.fsx_error_string.argtypes = [C.c_int]
+        L.fsx_error_string.restype = C.c_char_p
+        L.sxs_sbessel.argtypes = [C.c_int, C.c_double]
+        L.sxs_sbessel.restype = C.c_double
+        L.sxs_spf_full_create.argtypes = [C.c_int, C.c_int]
+        L.sxs_spf_full_create.restype = C.POINTER(SpfFull)
+        L.sxs_spf_full_free.argtypes = [C.POINTER(SpfFull)]
+        L.sxs_spf_full_read.argtypes = [C.c_char_p]
+        L.sxs_spf_full_read.restype = C.POINTER(SpfFull)
+        L.sxs_spf_full_write.argtypes = [C.c_char_p, C.POINTER(SpfFull)]
+        L.sxs_profile_create.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int]
+        L.sxs_profile_create.restype = C.POINTER(Profile)
+        L.sxs_profile_free.argtypes = [C.POINTER(Profile)]
+        L.sxs_profile_read.argtypes = [C.c_char_p]
+        L.sxs_profile_read.restype = C.POINTER(Profile)
+        L.sxs_profile_write.argtypes = [C.c_char_p, C.POINTER(Profile)]
+        L.sxs_profile_from_spf.argtypes = [C.POINTER(Profile), C.POINTER(SpfFull), C.c_double,
+                                           C.c_double]
+        L.sxs_opt_params_create.argtypes = [C.POINTER(Profile), C.POINTER(C.c_double), C.c_int,
+                                            C.c_double]
+        L.sxs_opt_params_create.restype = C.POINTER(OptParams)
+        L.sxs_opt_params_free.argtypes = [C.POINTER(OptParams)]
+        L.sxs_lbfgs_fitting.argtypes = [C.POINTER(Profile), C.POINTER(OptParams)]
+        L.sxs_spf2fitted_profile.argtypes = [C.POINTER(Profile), C.POINTER(SpfFull),
+                                             C.POINTER(OptParams)]
+        L.sxs_best_scale.argtypes = [C.POINTER(Profile), C.POINTER(OptParams), C.c_double,
+                                     C.c_double]
+        L.sxs_best_scale.restype = C.c_double
+        L.sxs_compute_saxs_scores.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_int, C.POINTER(SpfFull), C.POINTER(SpfFull),
+                                              C.POINTER(OptParams), C.c_void_p, C.c_int,
+                                              C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.sxs_assemble_index.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.sxs_assemble_index.restype = C.c_size_t
+        L.sxs_disassemble_index.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int]
+        L.sxs_mkarray.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.sxs_mkarray.restype = C.POINTER(C.c_double)
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _chk(rc):
+    if rc != 0:
+        raise FsxError(lib().fsx_error_string(rc).decode())
+
+
+def device_count():
+    return lib().fsx_device_count()
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class ScoringProblem:
+    """Flat-array scoring problem (fsx.h fsx_problem).  Arrays are kept alive
+    by this object."""
+
+    def __init__(self, L, qvals, zvals, A_re, A_im, B_re, B_im, a, mult, peak, nbeta=None,
+                 nalpha=None, ngamma=None, c1=(0.96, 1.04), c2=(-2.0, 4.0), pi=PI_REFERENCE):
+        self.keep = [f64(qvals), f64(zvals), f64(A_re).reshape(-1), f64(A_im).reshape(-1),
+                     f64(B_re).reshape(-1), f64(B_im).reshape(-1), f64(a)]
+        q, z, ar, ai, br, bi, aa = self.keep
+        K2 = (L + 1) ** 2
+        assert ar.size == 3 * q.size * K2 and br.size == ar.size and aa.size == 6 * q.size
+        self.s = Problem(L, nbeta or L + 1, nalpha or 2 * L + 1, ngamma or 2 * L + 1, q.size,
+                         _ptr(q), z.size, _ptr(z), _ptr(ar), _ptr(ai), _ptr(br), _ptr(bi),
+                         _ptr(aa), mult, peak, c1[0], c1[1], c2[0], c2[1], pi)
+
+    @property
+    def qnum(self):
+        return self.s.qnum
+
+
+def score(problem, poses, want_cross=False, out=None):
+    """Score poses (int64 mixed-radix indices).  Returns dict with score, c1,
+    c2 (NaN where the z slot is out of range), optional cross [n,6,Q], stats."""
+    poses = np.ascontiguousarray(poses, dtype=np.int64)
+    n = poses.size
+    sc = np.full(n, np.nan)
+    c1 = np.full(n, np.nan)
+    c2 = np.full(n, np.nan)
+    cross = np.full((n, 6, problem.qnum), np.nan) if want_cross else None
+    st = Stats()
+    _chk(lib().fsx_score(C.byref(problem.s), poses.ctypes.data, n, sc.ctypes.data,
+                         c1.ctypes.data, c2.ctypes.data,
+                         None if cross is None else cross.ctypes.data, 0, C.byref(st)))
+    return {"score": sc, "c1": c1, "c2": c2, "cross": cross, "stats": st.as_dict()}
+
+
+def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0):
+    """Device-resident form: all pointers are device addresses (ints)."""
+    st = Stats()
+    _chk(lib().fsx_score(C.byref(problem.s), d_poses, n, d_score, d_c1, d_c2, d_cross or None,
+                         FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT, C.byref(st)))
+    return st.as_dict()
+
+
+def fit_host(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0)):
+    ct = f64(ct)
+    out = np.zeros(4)
+    _chk(lib().fsx_fit_host(ct.reshape(-1), f64(qvals), ct.shape[-1], f64(a), mult, c1[0],
+                            c1[1], c2[0], c2[1], out))
+    return out
+
+
+def fit_batch(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0)):
+    """GPU fits of rescaled profiles ct [n,6,Q] -> [n,4] (score, c1, c2, evals)."""
+    ct = f64(ct)
+    n = ct.shape[0]
+    out = np.zeros((n, 4))
+    _chk(lib().fsx_fit_batch(ct.reshape(-1), n, f64(qvals), ct.shape[-1], f64(a), mult,
+                             c1[0], c1[1], c2[0], c2[1], out.reshape(-1)))
+    return out
+
+
+def sbessel_batch(x, pmax):
+    x = f64(x)
+    out = np.zeros((x.size, pmax + 1))
+    _chk(lib().fsx_sbessel_batch(x, x.size, pmax, out.reshape(-1)))
+    return out
+
+
+def sbessel(l, x):
+    return lib().sxs_sbessel(l, x)
+
+
+# ---- reference-struct helpers (drop-in API) -------------------------------------
+def spf_from_arrays(re, im, L, rm=0.0):
+    """Build a struct sxs_spf_full (owned by the library) from [3,Q,(L+1)^2]."""
+    re = f64(re)
+    im = f64(im)
+    Q = re.shape[1]
+    p = lib().sxs_spf_full_create(L, Q)
+    p.contents.rm = rm
+    K2 = (L + 1) ** 2
+    for q in range(Q):
+        for X, arr in enumerate((p.contents.V, p.contents.D, p.contents.W)):
+            s = arr[q].contents
+            C.memmove(s.re, re[X, q].ctypes.data, 8 * K2)
+            C.memmove(s.im, im[X, q].ctypes.data, 8 * K2)
+    return p
+
+
+def spf_to_arrays(p):
+    s = p.contents
+    L, Q = s.L, s.qnum
+    K2 = (L + 1) ** 2
+    re = np.zeros((3, Q, K2))
+    im = np.zeros((3, Q, K2))
+    for q in range(Q):
+        for X, arr in enumerate((s.V, s.D, s.W)):
+            re[X, q] = np.ctypeslib.as_array(arr[q].contents.re, shape=(K2,))
+            im[X, q] = np.ctypeslib.as_array(arr[q].contents.im, shape=(K2,))
+    return re, im, s.rm

@@ -1,0 +1,198 @@
+/*
+ * fsx_score.c -- host side of the scoring path: argument checks, the tables
+ * the GPU pipeline consumes (constant cross terms, Wigner-d per beta grid
+ * point, d-symbols, twiddles) and the drop-in sxs_compute_saxs_scores.
+ *
+ * Reference: src/fftsaxs.c:608-986.  The GPU evaluates the same cross-term
+ * profiles per requested pose with exact twiddles (the reference's dense-slab
+ * FFTW path, fftsaxs.c:899-900) instead of first summing whole (z, beta1,
+ * beta2) slabs; where the reference uses its truncated-pi direct DFT
+ * (slabs with < 30 poses, fftsaxs.c:529-606) the two differ by ~1e-12
+ * relative in I(q).
+ */
+#include "../fsx_internal.h"
+#include "fftsaxs.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+static double now_ms(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec * 1e3 + tv.tv_usec * 1e-3;
+}
+
+const char *fsx_error_string(int code)
+{
+    switch (code) {
+    case FSX_OK: return "ok";
+    case FSX_ERR_ARG: return "invalid argument";
+    case FSX_ERR_HIP: return "HIP runtime error";
+    case FSX_ERR_NOMEM: return "out of memory";
+    case FSX_ERR_NODEV: return "no HIP device";
+    default: return "unknown error";
+    }
+}
+
+int fsx_device_count(void) { return fsx_hip_device_count(); }
+void fsx_release(void) { fsx_hip_release(); }
+
+/* fftsaxs.c:27-50: sum_lm Re(A_X conj A_Y) + Re(B_X conj B_Y) per q */
+static void const_terms(const fsx_problem *pb, double *cint)
+{
+    static const int px[6] = {0, 0, 0, 1, 1, 2}, py[6] = {0, 1, 2, 1, 2, 2};
+    static const double fac[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};   /* fftsaxs.c:76-81 */
+    int K2 = (pb->L + 1) * (pb->L + 1), Q = pb->qnum;
+    size_t blk = (size_t)Q * K2;
+    for (int k = 0; k < 6; k++)
+        for (int q = 0; q < Q; q++) {
+            const double *a1r = pb->A_re + px[k] * blk + (size_t)q * K2;
+            const double *a1i = pb->A_im + px[k] * blk + (size_t)q * K2;
+            const double *a2r = pb->A_re + py[k] * blk + (size_t)q * K2;
+            const double *a2i = pb->A_im + py[k] * blk + (size_t)q * K2;
+            const double *b1r = pb->B_re + px[k] * blk + (size_t)q * K2;
+            const double *b1i = pb->B_im + px[k] * blk + (size_t)q * K2;
+            const double *b2r = pb->B_re + py[k] * blk + (size_t)q * K2;
+            const double *b2i = pb->B_im + py[k] * blk + (size_t)q * K2;
+            double v = 0.0;
+            for (int i = 0; i < K2; i++) {
+                v += a1r[i] * a2r[i] + a1i[i] * a2i[i];
+                v += b1r[i] * b2r[i] + b1i[i] * b2i[i];
+            }
+            cint[k * Q + q] = v * fac[k];
+        }
+}
+
+int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
+              double *c1, double *c2, double *cross, unsigned flags, fsx_stats *st)
+{
+    double t0 = now_ms();
+    fsx_stats local;
+    if (!st) st = &local;
+    memset(st, 0, sizeof(*st));
+    if (!pb || n < 0 || (n > 0 && (!poses || !score || !c1 || !c2))) return FSX_ERR_ARG;
+    if (pb->L < 1 || pb->L > 40 || pb->nbeta < 2 || pb->nalpha < 1 || pb->ngamma < 1 ||
+        pb->qnum < 2 || pb->znum < 1 || !pb->qvals || !pb->zvals || !pb->A_re ||
+        !pb->A_im || !pb->B_re || !pb->B_im || !pb->a)
+        return FSX_ERR_ARG;
+    if (n == 0) return FSX_OK;
+    int L = pb->L, K = L + 1, N = 2 * L + 1;
+    size_t dsz = (size_t)K * N * N;
+    const double *ds = fsx_d_symb_cached(L);
+    if (!ds) return FSX_ERR_NOMEM;
+    double *cint = (double *)malloc(sizeof(double) * 6 * pb->qnum);
+    double *dW = (double *)malloc(sizeof(double) * dsz * pb->nbeta);
+    double *twa = (double *)malloc(sizeof(double) * 2 * pb->nalpha);
+    double *twg = (double *)malloc(sizeof(double) * 2 * pb->ngamma);
+    if (!cint || !dW || !twa || !twg) {
+        free(cint); free(dW); free(twa); free(twg);
+        return FSX_ERR_NOMEM;
+    }
+    const_terms(pb, cint);
+    /* fftsaxs.c:632-636, 706-708: beta_i = 0 + i * (pi - 0)/(nbeta - 1) */
+    double bstep = (pb->pi - 0.0) / (pb->nbeta - 1);
+    for (int b = 0; b < pb->nbeta; b++) fsx_wigner_d(L, 0.0 + b * bstep, dW + b * dsz);
+    /* exact twiddles e^{-2 pi i k / N} (the FFTW transform of fftsaxs.c:680) */
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int k = 0; k < pb->nalpha; k++) {
+        twa[2 * k] = cos(two_pi * k / pb->nalpha);
+        twa[2 * k + 1] = -sin(two_pi * k / pb->nalpha);
+    }
+    for (int k = 0; k < pb->ngamma; k++) {
+        twg[2 * k] = cos(two_pi * k / pb->ngamma);
+        twg[2 * k + 1] = -sin(two_pi * k / pb->ngamma);
+    }
+    fsx_hip_args a;
+    memset(&a, 0, sizeof(a));
+    a.L = L; a.nbeta = pb->nbeta; a.nalpha = pb->nalpha; a.ngamma = pb->ngamma;
+    a.qnum = pb->qnum; a.znum = pb->znum; a.qvals = pb->qvals; a.zvals = pb->zvals;
+    a.A_re = pb->A_re; a.A_im = pb->A_im; a.B_re = pb->B_re; a.B_im = pb->B_im;
+    a.a = pb->a; a.cint = cint; a.dW = dW; a.dsymb = ds; a.tw_a = twa; a.tw_g = twg;
+    a.mult = pb->mult; a.peak = pb->peak;
+    a.c1_lo = pb->c1_lo; a.c1_hi = pb->c1_hi; a.c2_lo = pb->c2_lo; a.c2_hi = pb->c2_hi;
+    a.poses = poses; a.n = n; a.score = score; a.c1 = c1; a.c2 = c2; a.cross = cross;
+    a.flags = flags;
+    st->ms_tables = now_ms() - t0;
+    int rc = fsx_hip_run(&a, st);
+    free(cint); free(dW); free(twa); free(twg);
+    st->ms_total = now_ms() - t0;
+    return rc;
+}
+
+/* Flatten the reference structs into [3][qnum][(L+1)^2] arrays. */
+static void flatten_spf(const struct sxs_spf_full *s, int L, int qnum, double *re, double *im)
+{
+    int K2 = (L + 1) * (L + 1);
+    size_t blk = (size_t)qnum * K2;
+    int Ls = s->L < L ? s->L : L;
+    memset(re, 0, sizeof(double) * 3 * blk);
+    memset(im, 0, sizeof(double) * 3 * blk);
+    for (int q = 0; q < qnum; q++) {
+        struct sxs_spf_sing *x[3] = {s->V[q], s->D[q], s->W[q]};
+        for (int X = 0; X < 3; X++)
+            for (int i = 0; i < (Ls + 1) * (Ls + 1); i++) {
+                re[X * blk + (size_t)q * K2 + i] = x[X]->re[i];
+                im[X * blk + (size_t)q * K2 + i] = x[X]->im[i];
+            }
+    }
+}
+
+void sxs_compute_saxs_scores(double *scores_list, double *c1_list, double *c2_list,
+                             int *index_list, int nout, struct sxs_spf_full *A,
+                             struct sxs_spf_full *B, struct sxs_opt_params *params,
+                             double *qvals, int qnum, double *zvals, int znum, int L,
+                             int skip)
+{
+    (void)skip;  /* skip = 0 only changes the reference's transform choice (exact twiddles) */
+    if (!A || !B || !params || !qvals || !zvals || !index_list || !scores_list || !c1_list ||
+        !c2_list) {
+        fprintf(stderr, "[Error] sxs_compute_saxs_scores: Null pointer\n");
+        exit(EXIT_FAILURE);
+    }
+    int K2 = (L + 1) * (L + 1);
+    size_t blk = (size_t)qnum * K2;
+    double *buf = (double *)malloc(sizeof(double) * 12 * blk);
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nout > 0 ? nout : 1));
+    if (!buf || !idx) {
+        fprintf(stderr, "[Error] sxs_compute_saxs_scores: out of memory\n");
+        exit(EXIT_FAILURE);
+    }
+    flatten_spf(A, L, qnum, buf, buf + 3 * blk);
+    flatten_spf(B, L, qnum, buf + 6 * blk, buf + 9 * blk);
+    for (int i = 0; i < nout; i++) idx[i] = index_list[i];
+    fsx_problem pb;
+    memset(&pb, 0, sizeof(pb));
+    pb.L = L; pb.nbeta = L + 1; pb.nalpha = 2 * L + 1; pb.ngamma = 2 * L + 1;
+    pb.qnum = qnum; pb.qvals = qvals; pb.znum = znum; pb.zvals = zvals;
+    pb.A_re = buf; pb.A_im = buf + 3 * blk; pb.B_re = buf + 6 * blk; pb.B_im = buf + 9 * blk;
+    pb.a = params->a; pb.mult = params->mult; pb.peak = params->peak;
+    pb.c1_lo = 0.96; pb.c1_hi = 1.04; pb.c2_lo = -2.00; pb.c2_hi = 4.00;
+    pb.pi = FSX_PI_REFERENCE;
+    int rc = fsx_score(&pb, idx, nout, scores_list, c1_list, c2_list, NULL, 0, NULL);
+    free(buf);
+    free(idx);
+    if (rc != FSX_OK) {
+        fprintf(stderr, "[Error] sxs_compute_saxs_scores: %s\n", fsx_error_string(rc));
+        exit(EXIT_FAILURE);
+    }
+}
+
+int fsx_fit_batch(const double *ct, int n, const double *qvals, int qnum, const double *a,
+                  double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                  double *out)
+{
+    if (n < 0 || (n > 0 && (!ct || !qvals || !a || !out)) || qnum < 2) return FSX_ERR_ARG;
+    if (n == 0) return FSX_OK;
+    return fsx_hip_fit_batch(ct, n, qvals, qnum, a, mult, c1_lo, c1_hi, c2_lo, c2_hi, out);
+}
+
+int fsx_sbessel_batch(const double *x, int n, int pmax, double *out)
+{
+    if (n < 0 || pmax < 0 || (n > 0 && (!x || !out))) return FSX_ERR_ARG;
+    if (n == 0) return FSX_OK;
+    return fsx_hip_sbessel_batch(x, n, pmax, out);
+}
