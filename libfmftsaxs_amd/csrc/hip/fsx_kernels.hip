@@ -52,6 +52,32 @@ __device__ __forceinline__ int pos_mod(long v, int n)
     return (int)(r < 0 ? r + n : r);
 }
 
+/* x^n for x > 0, integer n >= 0, correctly rounded: binary exponentiation
+ * in double-double (each product exact to ~2^-104 through fma), rounded
+ * once at the end.  glibc's pow, which the reference calls, is correctly
+ * rounded for these arguments in practice; the device pow is not. */
+__device__ __forceinline__ void dd_mul(double ah, double al, double bh, double bl, double *rh,
+                                       double *rl)
+{
+    double p = ah * bh;
+    double e = fma(ah, bh, -p);
+    e = e + (ah * bl + al * bh);
+    double h = p + e;
+    *rl = e - (h - p);
+    *rh = h;
+}
+
+__device__ double pow_int_cr(double x, int n)
+{
+    double rh = 1.0, rl = 0.0, bh = x, bl = 0.0;
+    while (n) {
+        if (n & 1) dd_mul(rh, rl, bh, bl, &rh, &rl);
+        n >>= 1;
+        if (n) dd_mul(bh, bl, bh, bl, &bh, &bl);
+    }
+    return rh + rl;
+}
+
 /* j_l(x) exactly as sfbessel.c:40-60 (contraction off) */
 __device__ double dev_sbessel(int l, double x)
 {
@@ -68,7 +94,7 @@ __device__ double dev_sbessel(int l, double x)
             term *= ratio;
             sum += term;
         }
-        return sum * pow(x, (double)l);
+        return sum * pow_int_cr(x, l);
     }
     return l == 0 ? 1.0 : 0.0;
 }

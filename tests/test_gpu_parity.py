@@ -1,0 +1,165 @@
+"""GPU parity of the HIP scoring path against the oracle (FP64 CPU
+restatement of the reference).  Bars, stated per check:
+  * Bessel table, fit of identical profiles: bit-exact except where the
+    device and host libm (pow, exp) round differently (counted, <= 2 ulp);
+  * cross-term profiles / I(q): within 1e-10 relative of the oracle
+    (north star: I(q) within 1e-10 relative);
+  * score, c1, c2: within 1e-7 relative / 1e-6 absolute and identical
+    score ranking of the top poses.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problem import build_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def F():
+    import libfmftsaxs_amd as F
+    if F.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return F
+
+
+def _I(c):
+    return c[..., 0, :] - c[..., 1, :] + c[..., 2, :] + c[..., 3, :] - c[..., 4, :] + c[..., 5, :]
+
+
+def _cmp_results(pb, r_gpu, r_orc, I_tol=1e-10):
+    ok = ~np.isnan(r_orc["score"])
+    assert np.array_equal(ok, ~np.isnan(r_gpu["score"]))
+    cg, co = r_gpu["cross"][ok], r_orc["cross"][ok]
+    Ig, Io = _I(cg), _I(co)
+    relI = np.abs(Ig - Io) / np.abs(Io)
+    assert relI.max() <= I_tol, relI.max()
+    scale = np.abs(co).max(axis=2, keepdims=True)
+    assert (np.abs(cg - co) / scale).max() <= I_tol
+    s_g, s_o = r_gpu["score"][ok], r_orc["score"][ok]
+    assert np.all(np.abs(s_g - s_o) <= 1e-7 * np.abs(s_o))
+    assert np.all(np.abs(r_gpu["c1"][ok] - r_orc["c1"][ok]) <= 1e-6)
+    assert np.all(np.abs(r_gpu["c2"][ok] - r_orc["c2"][ok]) <= 1e-6)
+    k = min(100, ok.sum())
+    assert np.array_equal(np.argsort(s_g, kind="stable")[:k], np.argsort(s_o, kind="stable")[:k])
+    return relI.max()
+
+
+def _gpu_problem(F, pb, **kw):
+    return F.ScoringProblem(pb["L"], pb["qvals"], pb["zvals"], pb["A_re"], pb["A_im"], pb["B_re"],
+                            pb["B_im"], pb["a"], pb["params"].mult, pb["peak"],
+                            nbeta=pb["nbeta"], nalpha=pb["na"], ngamma=pb["ng"], **kw)
+
+
+def test_sbessel_table(F):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(0, 45, 3000), [0.0, 1e-9, 40.0]])
+    g = F.sbessel_batch(x, 40)
+    o = np.array([[O.sbessel(p, xi) for p in range(41)] for xi in x])
+    ulp = np.abs(g - o) / np.spacing(np.abs(o) + 1e-300)
+    print("sbessel exact:", np.mean(g == o), "max ulp", ulp.max())
+    assert ulp.max() <= 2
+    assert np.mean(g == o) > 0.99
+
+
+def test_fit_identical_profiles(F):
+    pb = build_problem(300, 200, 8, 0.3, 25, np.arange(15.0, 35.0), 400, seed=21)
+    r = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                              pb["params"], pb["qvals"], pb["zvals"], pb["L"], want_cross=True)
+    cts, want = [], []
+    for ct in r["cross"]:
+        sc, c1, c2, nev, cs = O.fit_one(ct, pb["qvals"], pb["params"])
+        cts.append(cs)
+        want.append((sc, c1, c2, nev))
+    want = np.array(want)
+    got = F.fit_batch(np.array(cts), pb["qvals"], pb["a"], pb["params"].mult)
+    exact = np.all(got == want, axis=1)
+    print("bit-exact fits:", exact.mean())
+    assert exact.mean() >= 0.95
+    assert np.all(np.abs(got[:, 0] - want[:, 0]) <= 1e-9 * want[:, 0])
+    assert np.all(np.abs(got[:, 1:3] - want[:, 1:3]) <= 1e-6)
+
+
+@pytest.mark.parametrize("L,Q,qmax,nat", [(6, 21, 0.3, (300, 200)), (10, 31, 0.3, (600, 400)),
+                                          (15, 50, 0.5, (800, 500))])
+def test_native_grid_vs_oracle(F, L, Q, qmax, nat):
+    zv = np.arange(18.0, 30.0)
+    pb = build_problem(nat[0], nat[1], L, qmax, Q, zv, 150, seed=L)
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L, want_cross=True)
+    gp = _gpu_problem(F, pb)
+    r_g = F.score(gp, pb["index"], want_cross=True)
+    rel = _cmp_results(pb, r_g, r_o)
+    print(f"L={L}: max rel I diff {rel:.2e}; stats {r_g['stats']}")
+
+
+def test_extended_grid_vs_oracle(F):
+    """nbeta = 19 (10 deg), 64-point alpha/gamma grids: beyond the reference's
+    grid; the oracle evaluates the same generalised transform directly."""
+    L = 8
+    zv = np.arange(25.0, 31.0)
+    pb = build_problem(400, 300, L, 0.5, 21, zv, 60, seed=5, nbeta=19, na=64, ng=64)
+    O.set_pi(O.PI_TRUNC)
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L, nbeta=19, na=64, ng=64,
+                                want_cross=True)
+    r_g = F.score(_gpu_problem(F, pb), pb["index"], want_cross=True)
+    _cmp_results(pb, r_g, r_o)
+
+
+def test_out_of_range_poses_untouched(F):
+    L = 6
+    zv = np.arange(20.0, 26.0)
+    pb = build_problem(200, 150, L, 0.3, 17, zv, 30, seed=8)
+    N = 2 * L + 1
+    size5d = (L + 1) ** 2 * N ** 3
+    idx = np.concatenate([pb["index"], [len(zv) * size5d + 5, -3]]).astype(np.int64)
+    r = F.score(_gpu_problem(F, pb), idx)
+    assert np.all(np.isnan(r["score"][-2:])) and np.all(np.isnan(r["c1"][-2:]))
+    assert not np.any(np.isnan(r["score"][:-2]))
+    assert r["stats"]["n_scored"] == len(pb["index"])
+
+
+def test_duplicate_poses_identical(F):
+    L = 6
+    zv = np.arange(20.0, 26.0)
+    pb = build_problem(200, 150, L, 0.3, 17, zv, 20, seed=9)
+    idx = np.concatenate([pb["index"], pb["index"][:7], pb["index"][:3]])
+    r = F.score(_gpu_problem(F, pb), idx)
+    assert np.array_equal(r["score"][20:27], r["score"][:7])
+    assert np.array_equal(r["c2"][27:], r["c2"][:3])
+
+
+def test_dropin_sxs_compute_saxs_scores(F):
+    """The reference-signature entry point on reference structs."""
+    import ctypes as C
+    L = 6
+    zv = np.arange(20.0, 26.0)
+    pb = build_problem(200, 150, L, 0.3, 17, zv, 25, seed=10)
+    lib = F.lib()
+    A = F.spf_from_arrays(pb["A_re"], pb["A_im"], L)
+    B = F.spf_from_arrays(pb["B_re"], pb["B_im"], L)
+    exp_q = np.ascontiguousarray(pb["exp_q"])
+    prof = lib.sxs_profile_create(exp_q.ctypes.data_as(C.POINTER(C.c_double)), exp_q.size, 0)
+    C.memmove(prof.contents.in_, pb["exp_in"].ctypes.data, 8 * exp_q.size)
+    C.memmove(prof.contents.err, pb["exp_err"].ctypes.data, 8 * exp_q.size)
+    q = np.ascontiguousarray(pb["qvals"])
+    params = lib.sxs_opt_params_create(prof, q.ctypes.data_as(C.POINTER(C.c_double)), q.size,
+                                       pb["rm"])
+    a = np.ctypeslib.as_array(params.contents.a, shape=(6 * q.size,))
+    assert np.array_equal(a, pb["a"])
+    n = len(pb["index"])
+    idx = np.ascontiguousarray(pb["index"], dtype=np.int32)
+    sc, c1, c2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    z = np.ascontiguousarray(zv)
+    lib.sxs_compute_saxs_scores(sc.ctypes.data, c1.ctypes.data, c2.ctypes.data, idx.ctypes.data,
+                                n, A, B, params, q.ctypes.data, q.size, z.ctypes.data, z.size, L, 1)
+    r = F.score(_gpu_problem(F, pb), pb["index"])
+    assert np.array_equal(sc, r["score"]) and np.array_equal(c1, r["c1"])
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L)
+    assert np.all(np.abs(sc - r_o["score"]) <= 1e-7 * r_o["score"])
+    lib.sxs_opt_params_free(params)
+    lib.sxs_spf_full_free(A)
+    lib.sxs_spf_full_free(B)
