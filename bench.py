@@ -1,0 +1,177 @@
+"""Benchmark: poses scored per second, including the per-pose c1/c2 fit, of
+the FFT dimer SAXS scoring path on one MI355X, FP64 form.
+
+Workload (one step = one scoring pass over the whole pose set, inputs
+resident in HBM): synthetic receptor of 3,000 atoms and ligand of 1,500
+(compact random-walk chains, C/N/O/S form factors, Shrake-Rupley SASA),
+L = 15, 51 q on [0, 0.5] 1/A, z = 25..60 A in 1 A steps (36), beta_rec /
+beta_lig in 10 degree steps (19 points), 64-point alpha_lig / gamma grids,
+210,000 distinct grid poses, c1 in [0.95, 1.05], c2 in [0, 4].  The
+'experimental' curve is the first pose's I(q) at c1 = c2 = 1 plus 1 %
+Gaussian noise (sigma = 5 % I).
+
+Prints one JSON line (see DESIGN.md for the roofline accounting).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60,
+                   nbeta=19, ngrid=64, n_rec=3000, n_lig=1500, c1=(0.95, 1.05), c2=(0.0, 4.0)):
+    import libfmftsaxs_amd as F
+    from libfmftsaxs_amd import synthetic as S
+    t0 = time.time()
+    rec, lig = S.make_dimer_inputs(n_rec, n_lig, 1001 + seed, 2001 + seed)
+    q = F.mkarray(0.0, qmax, qnum)
+    Ar, Ai, rmA = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
+    Br, Bi, rmB = F.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
+    zv = np.arange(float(zlo), float(zhi) + 0.5, 1.0)
+    poses = S.random_grid_poses(npose, zv.size, nbeta, ngrid, ngrid, 3001 + seed)
+    rm = (rmA * rec.natoms + rmB * lig.natoms) / (rec.natoms + lig.natoms)
+    # experimental curve from the first pose
+    probe = F.ScoringProblem(L, q, zv, Ar, Ai, Br, Bi, np.ones(6 * qnum), 1.0, 1.0,
+                             nbeta=nbeta, nalpha=ngrid, ngamma=ngrid, c1=c1, c2=c2)
+    ct = F.score(probe, poses[:1], want_cross=True)["cross"][0]
+    I = ct[0] - ct[1] + ct[2] + ct[3] - ct[4] + ct[5]
+    rng = np.random.default_rng(4001 + seed)
+    Iexp = I * (1 + 0.01 * rng.normal(size=qnum))
+    a, mult, peak = F.fit_params(q, Iexp, 0.05 * Iexp, q, rm)
+    pb = F.ScoringProblem(L, q, zv, Ar, Ai, Br, Bi, a, mult, peak, nbeta=nbeta, nalpha=ngrid,
+                          ngamma=ngrid, c1=c1, c2=c2)
+    log(f"workload built in {time.time() - t0:.1f} s")
+    return dict(problem=pb, poses=poses, q=q, zv=zv, L=L, A=(Ar, Ai), B=(Br, Bi), rm=rm,
+                a=a, mult=mult, peak=peak, exp=(q, Iexp, 0.05 * Iexp), nbeta=nbeta,
+                ngrid=ngrid)
+
+
+def cpu_baseline(w, max_threads=16):
+    """The oracle (FP64 CPU restatement of the reference algorithm: per-slab
+    second sums, DFT to the Euler grid, one L-BFGS-B fit per pose) on a
+    bounded sample: the same molecules and q grid on the reference's own
+    native grid (L+1 betas, 2L+1-point alpha/gamma), the poses of one z plane
+    (z = 40 A) with receptor beta index < 4, from a 210,000-pose native-grid
+    set."""
+    import oracle as O
+    O.set_pi(O.PI_TRUNC)
+    L = w["L"]
+    N = 2 * L + 1
+    from libfmftsaxs_amd import synthetic as S
+    zv = w["zv"]
+    idx = S.random_grid_poses(210000, zv.size, L + 1, N, N, 5001)
+    size5d = (L + 1) ** 2 * N ** 3
+    zsel = int(np.argmin(np.abs(zv - 40.0)))
+    b1 = (idx % size5d) // ((L + 1) * N ** 3)
+    sample = idx[(idx // size5d == zsel) & (b1 < 4)]
+    P = O.Params(w["a"], w["rm"], w["peak"])
+    P.s.mult = w["mult"]
+    nthr = max(1, min(max_threads, os.cpu_count() or 1))
+    t = time.time()
+    r = O.compute_saxs_scores(sample, w["A"][0], w["A"][1], w["B"][0], w["B"][1], P, w["q"],
+                              zv, L, nthreads=nthr)
+    dt = time.time() - t
+    assert not np.any(np.isnan(r["score"]))
+    return {"value": len(sample) / dt, "unit": "poses/s", "cores": nthr, "kind": "port",
+            "sample": f"{len(sample)} poses: those of one z plane (z=40 A) with beta_rec index < 4 "
+                      f"from a 210000-pose "
+                      f"native-grid set (L=15, 16 betas, 31-point alpha/gamma), same "
+                      f"molecules and 51 q; oracle OpenMP over slabs, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--poses", type=int, default=210000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build targets single-GPU boxes "
+              "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import libfmftsaxs_amd as F
+    if F.device_count() < 1:
+        raise SystemExit("no HIP device")
+    w = build_workload(npose=args.poses)
+    pb = w["problem"]
+    n = len(w["poses"])
+    dev = torch.device("cuda:0")
+    d_poses = torch.from_numpy(w["poses"]).to(dev)
+    d_score = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    d_c1 = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    d_c2 = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        return F.score_device(pb, d_poses.data_ptr(), n, d_score.data_ptr(), d_c1.data_ptr(),
+                              d_c2.data_ptr())
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: {st['ms_total']:.1f} ms")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(step())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+    value = n / (dt / args.steps)
+    assert int(torch.isnan(d_score).sum()) == 0
+    keys = ["ms_tables", "ms_bessel_t", "ms_rotate", "ms_group", "ms_cross", "ms_fit"]
+    avg = {k: float(np.mean([s[k] for s in stats])) for k in keys}
+    log("per-stage ms: " + ", ".join(f"{k[3:]}={v:.3f}" for k, v in avg.items()))
+    cb = float(np.mean([s["cross_bytes"] for s in stats]))
+    # dominant kernel and its roofline
+    if avg["ms_cross"] >= avg["ms_fit"]:
+        dom = "k_cross"
+        t_k = avg["ms_cross"] * 1e-3
+        achieved = cb / t_k / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
+                "ms_per_launch": avg["ms_cross"], "algorithmic_bytes": cb}
+    else:
+        dom = "k_fit"
+        t_k = avg["ms_fit"] * 1e-3
+        fb = n * (6 * pb.qnum * 8 + 24 + 4)
+        achieved = fb / t_k / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
+                "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(w)
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed:", repr(e))
+    out = {"metric": "poses scored/sec incl. c1,c2 fit (FP64 GPU form), 1 GPU",
+           "value": value, "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "config 1: receptor 3000 / ligand 1500 atoms, L=15, 51 q "
+                                  "on [0,0.5], z 25..60 step 1, 19 betas (10 deg), 64-point "
+                                  "alpha/gamma grids, 210000 poses, c1 [0.95,1.05], "
+                                  "c2 [0,4], FP64",
+                      "poses": n, "L": w["L"], "qnum": int(pb.qnum), "nbeta": w["nbeta"],
+                      "ngrid": w["ngrid"], "znum": int(w["zv"].size)},
+           "stage_ms": avg, "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -99,6 +99,17 @@ int fsx_fit_batch(const double *ct, int n, const double *qvals, int qnum, const 
                   double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
                   double *out);
 
+/* Spherical-harmonic amplitude expansion of an atom group, the array form of
+ * atom_grp2spf_inplace (src/pdb2spf.c:24-152): coords [natoms][3] already
+ * centred; vac/dum per-atom scalar form factors; sa per-atom SASA fraction
+ * (or NULL: no water shell) weighted by h2o_ff (zero_ff of OH2, 3.50968);
+ * vdw per-atom radii (their mean is returned in *rm).  Output re/im
+ * [3][qnum][(L+1)^2] (V, D, W) on the host.  Bessel values and atom sums run
+ * on the GPU. */
+int fsx_atoms2spf(int natoms, const double *coords, const double *vac, const double *dum,
+                  const double *sa, double h2o_ff, const double *vdw, const double *qvals,
+                  int qnum, int L, double *re, double *im, double *rm);
+
 /* Bessel table j_p(x_i) for p = 0..pmax on the GPU, x host [n]; out [n][pmax+1] */
 int fsx_sbessel_batch(const double *x, int n, int pmax, double *out);
 

@@ -87,6 +87,9 @@ def lib():
         L.fsx_fit_batch.restype = C.c_int
         L.fsx_sbessel_batch.argtypes = [_dp, C.c_int, C.c_int, _dp]
         L.fsx_sbessel_batch.restype = C.c_int
+        L.fsx_atoms2spf.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double, _dp, _dp,
+                                    C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
+        L.fsx_atoms2spf.restype = C.c_int
         L.fsx_device_count.restype = C.c_int
         L.fsx_error_string.argtypes = [C.c_int]
         L.fsx_error_string.restype = C.c_char_p
@@ -215,8 +218,50 @@ def sbessel_batch(x, pmax):
     return out
 
 
+def atoms2spf(coords, vac, dum, sa, vdw, qvals, L, h2o_ff=3.50968):
+    """GPU atom expansion (pdb2spf.c:24-152).  Returns (re, im, rm) with
+    re/im shaped [3, Q, (L+1)^2]."""
+    coords = f64(coords).reshape(-1, 3)
+    qvals = f64(qvals)
+    re = np.zeros((3, qvals.size, (L + 1) ** 2))
+    im = np.zeros_like(re)
+    rm = C.c_double(0.0)
+    sa_arr = None if sa is None else f64(sa)
+    _chk(lib().fsx_atoms2spf(coords.shape[0], coords.ravel(), f64(vac), f64(dum),
+                             None if sa_arr is None else sa_arr.ctypes.data, h2o_ff, f64(vdw),
+                             qvals, qvals.size, L, re.reshape(-1), im.reshape(-1), C.byref(rm)))
+    return re, im, rm.value
+
+
 def sbessel(l, x):
     return lib().sxs_sbessel(l, x)
+
+
+def mkarray(begin, end, n):
+    """saxs_utils.c:50-63 q grid (cumulative addition, H3) from the library."""
+    L = lib()
+    p = L.sxs_mkarray(begin, end, n)
+    out = np.ctypeslib.as_array(p, shape=(n,)).copy()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free(C.cast(p, C.c_void_p))
+    return out
+
+
+def fit_params(exp_q, exp_in, exp_err, qvals, rm):
+    """sxs_opt_params_create (min_saxs.c:108-124) for an experimental curve
+    given as arrays: returns (a [6*Q], mult, peak)."""
+    L = lib()
+    eq, ei, ee, q = f64(exp_q), f64(exp_in), f64(exp_err), f64(qvals)
+    prof = L.sxs_profile_create(_ptr(eq), eq.size, 0)
+    C.memmove(prof.contents.in_, ei.ctypes.data, 8 * eq.size)
+    C.memmove(prof.contents.err, ee.ctypes.data, 8 * eq.size)
+    par = L.sxs_opt_params_create(prof, _ptr(q), q.size, rm)
+    a = np.ctypeslib.as_array(par.contents.a, shape=(6 * q.size,)).copy()
+    mult, peak = par.contents.mult, par.contents.peak
+    L.sxs_opt_params_free(par)
+    L.sxs_profile_free(prof)
+    return a, mult, peak
 
 
 # ---- reference-struct helpers (drop-in API) -------------------------------------

@@ -412,6 +412,63 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     out[4L * s + 3] = nev;
 }
 
+/* ---- atom expansion (pdb2spf.c:24-152) ------------------------------------ */
+__global__ void k_atom_bessel(double *J, const double *r, const double *qvals, int natoms, int Q,
+                              int K)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)natoms * Q * K) return;
+    int a = (int)(i % natoms);
+    int l = (int)((i / natoms) % K);
+    int q = (int)(i / ((long)natoms * K));
+    J[i] = dev_sbessel(l, qvals[q] * r[a]);
+}
+
+/* one block per (q, lm): sum over atoms of f^X_a j_l(q r_a) conj(Y_lm), then
+ * the 4 pi i^l factor (pdb2spf.c:118-147) */
+__global__ void __launch_bounds__(256)
+k_atom_sum(double *re, double *im, const double *J, const double *Y, const double *f,
+           int natoms, int Q, int L, double four_pi)
+{
+    __shared__ double red[6][4];
+    const int K = L + 1, K2 = K * K;
+    int lm = blockIdx.x % K2, q = blockIdx.x / K2;
+    int l = lm_l(lm);
+    const double *Jq = J + ((long)q * K + l) * natoms;
+    const double *yr = Y + (long)lm * natoms, *yi = Y + (long)(K2 + lm) * natoms;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
+        double j = Jq[a];
+        double vr = j * yr[a], vi = j * yi[a];
+#pragma unroll
+        for (int X = 0; X < 3; X++) {
+            double fx = f[(long)X * natoms + a];
+            acc[2 * X] += fx * vr;
+            acc[2 * X + 1] += fx * vi;
+        }
+    }
+    int wave = threadIdx.x / FSX_WAVE, lane = threadIdx.x % FSX_WAVE;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double v = wave_sum(acc[k]);
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int X = threadIdx.x;
+        double vr = 0.0, vi = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / FSX_WAVE); w++) {
+            vr += red[2 * X][w];
+            vi += red[2 * X + 1][w];
+        }
+        const double cr[4] = {four_pi, 0.0, -four_pi, 0.0}, ci[4] = {0.0, four_pi, 0.0, -four_pi};
+        double c_r = cr[l % 4], c_i = ci[l % 4];
+        long o = ((long)X * Q + q) * K2 + lm;
+        re[o] = c_r * vr - c_i * vi;
+        im[o] = c_r * vi + c_i * vr;
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* host side: device buffers, stream, launches */
 
@@ -421,7 +478,8 @@ static hipStream_t g_stream;
 enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_SORTED, B_CROSS2, B_SCORE, B_C1, B_C2,
-    B_CROSSOUT, B_NEV, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_NBUF
+    B_CROSSOUT, B_NEV, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF, B_AY, B_AJ, B_AQ,
+    B_ARE, B_AIM, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -712,6 +770,49 @@ extern "C" int fsx_hip_sbessel_batch(const double *x, int n, int pmax, double *o
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             fprintf(stderr, "[fsx] sbessel batch: %s\n", hipGetErrorString(e));
+            rc = FSX_ERR_HIP;
+        }
+    }
+    pthread_mutex_unlock(&g_mtx);
+    return rc;
+}
+
+extern "C" int fsx_hip_atoms2spf(int natoms, const double *r, const double *f, const double *Y,
+                                 const double *qvals, int Q, int L, double *re, double *im)
+{
+    pthread_mutex_lock(&g_mtx);
+    const int K = L + 1, K2 = K * K;
+    int rc = init_once();
+    if (rc == FSX_OK) rc = ensure(B_AR, sizeof(double) * natoms);
+    if (rc == FSX_OK) rc = ensure(B_AF, sizeof(double) * 3 * (size_t)natoms);
+    if (rc == FSX_OK) rc = ensure(B_AY, sizeof(double) * 2 * (size_t)K2 * natoms);
+    if (rc == FSX_OK) rc = ensure(B_AJ, sizeof(double) * (size_t)Q * K * natoms);
+    if (rc == FSX_OK) rc = ensure(B_AQ, sizeof(double) * Q);
+    if (rc == FSX_OK) rc = ensure(B_ARE, sizeof(double) * 3 * (size_t)Q * K2);
+    if (rc == FSX_OK) rc = ensure(B_AIM, sizeof(double) * 3 * (size_t)Q * K2);
+    if (rc == FSX_OK) {
+        hipStream_t s = g_stream;
+        hipError_t e = hipMemcpyAsync(g_buf[B_AR], r, sizeof(double) * natoms, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AF], f, sizeof(double) * 3 * (size_t)natoms, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AY], Y, sizeof(double) * 2 * (size_t)K2 * natoms, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AQ], qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            long nj = (long)Q * K * natoms;
+            hipLaunchKernelGGL(k_atom_bessel, dim3(nblk(nj, 256)), dim3(256), 0, s,
+                               (double *)g_buf[B_AJ], (const double *)g_buf[B_AR],
+                               (const double *)g_buf[B_AQ], natoms, Q, K);
+            hipLaunchKernelGGL(k_atom_sum, dim3((unsigned)(Q * K2)), dim3(256), 0, s,
+                               (double *)g_buf[B_ARE], (double *)g_buf[B_AIM],
+                               (const double *)g_buf[B_AJ], (const double *)g_buf[B_AY],
+                               (const double *)g_buf[B_AF], natoms, Q, L,
+                               4.0 * FSX_PI_REFERENCE);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(re, g_buf[B_ARE], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(im, g_buf[B_AIM], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            fprintf(stderr, "[fsx] atoms2spf: %s\n", hipGetErrorString(e));
             rc = FSX_ERR_HIP;
         }
     }

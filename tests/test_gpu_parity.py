@@ -163,3 +163,20 @@ def test_dropin_sxs_compute_saxs_scores(F):
     lib.sxs_opt_params_free(params)
     lib.sxs_spf_full_free(A)
     lib.sxs_spf_full_free(B)
+
+
+@pytest.mark.parametrize("L", [6, 15])
+def test_atoms2spf_vs_oracle(F, L):
+    """GPU atom expansion (pdb2spf.c:24-152) against the oracle restatement;
+    only the order of the sum over atoms differs."""
+    from libfmftsaxs_amd import synthetic as S
+    rec = S.random_chain(700, 17)
+    rec = rec.translated(-rec.center_of_extrema())
+    rec.sa = S.shrake_rupley(rec)
+    q = O.mkarray(0.0, 0.5, 26)
+    rg, ig, rmg = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
+    ro, io, rmo = O.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
+    assert rmg == rmo
+    scale = np.maximum(np.abs(ro).max(axis=2, keepdims=True), np.abs(io).max(axis=2, keepdims=True))
+    assert np.all(np.abs(rg - ro) <= 1e-12 * scale)
+    assert np.all(np.abs(ig - io) <= 1e-12 * scale)
