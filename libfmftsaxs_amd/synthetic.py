@@ -78,9 +78,11 @@ def random_chain(natoms, seed):
     return Molecule(coords, elements)
 
 
-def shrake_rupley(mol, probe=1.4, npts=96, seed=7):
-    """Fraction of accessible surface per atom, normalised like sxs_faccs
-    (saxs_utils.c:30-48): accessible area / (4 pi r_vdw^2)."""
+def shrake_rupley(mol, probe=1.4, npts=96, seed=7, occluders="all"):
+    """Fraction of solvent-accessible surface per atom (Shrake-Rupley, probe
+    1.4 A, Fibonacci sphere of npts points on r_vdw + probe), in the
+    normalisation sxs_faccs returns (saxs_utils.c:30-48): contact area over
+    4 pi r_vdw^2, i.e. the unburied fraction of the probe sphere."""
     rng = np.random.default_rng(seed)
     # Fibonacci sphere
     k = np.arange(npts) + 0.5
@@ -105,7 +107,8 @@ def shrake_rupley(mol, probe=1.4, npts=96, seed=7):
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
                     nb.extend(grid.get((kx + dx, ky + dy, kz + dz), []))
-        nb = np.array([j for j in nb if j != i], dtype=np.int64)
+        nb = np.array([j for j in nb if j != i and (occluders == "all" or mol.vdw[j] > 0)],
+                      dtype=np.int64)
         pts = X[i] + R[i] * sph
         if nb.size:
             d2 = ((pts[:, None, :] - X[nb][None, :, :]) ** 2).sum(-1)
@@ -113,8 +116,11 @@ def shrake_rupley(mol, probe=1.4, npts=96, seed=7):
             free = npts - buried.sum()
         else:
             free = npts
-        area = 4 * np.pi * R[i] ** 2 * free / npts
-        sa[i] = area / (4 * np.pi * mol.vdw[i] ** 2)
+        ri = mol.vdw[i]
+        # contact surface (the accessible fraction scaled to the vdW sphere,
+        # sxs_faccs calls accs with cont_acc = 1), divided by 4 pi r^2 as in
+        # saxs_utils.c:40-47; radius-0 atoms (hydrogens) carry no SASA
+        sa[i] = 0.0 if ri == 0 else free / npts
     return sa
 
 
