@@ -10,7 +10,8 @@ beta_lig in 10 degree steps (19 points), 64-point alpha_lig / gamma grids,
 'experimental' curve is the first pose's I(q) at c1 = c2 = 1 plus 1 %
 Gaussian noise (sigma = 5 % I).
 
-Prints one JSON line (see DESIGN.md for the roofline accounting).
+Prints one JSON line with the throughput, the roofline position of the
+dominant kernel and a CPU baseline.
 """
 import argparse
 import json
@@ -99,7 +100,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build targets single-GPU boxes "
-              "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
+              "(the path shards by z / (z, beta2) buckets with no exchange step)", file=sys.stderr)
         sys.exit(2)
 
     import torch

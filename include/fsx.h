@@ -6,7 +6,8 @@
  * calls, generalised to
  *   - any beta grid size (beta_i = i*pi/(nbeta-1); the reference fixes L+1),
  *   - any alpha_B and gamma_A/gamma_B grid sizes (reference: 2L+1),
- *   - 64-bit pose indices (the reference's int index overflows, H7),
+ *   - 64-bit pose indices (the reference's int index overflows for L >= 20
+ *     with 80 z values),
  *   - caller-chosen c1/c2 bounds (reference: define.h:28-31),
  *   - inputs and outputs in host or device (HBM) memory,
  * and it returns error codes instead of exiting.
@@ -42,7 +43,8 @@ extern "C" {
 #define FSX_DEVICE_OUTPUT 2u   /* score/c1/c2/cross point to device memory */
 #define FSX_ALL_POSES_VALID 4u /* caller guarantees every index is in range */
 
-/* pi of the reference CMake build (-std=c11: define.h:13-15, H1) */
+/* pi of the reference CMake build (-std=c11 leaves M_PI undefined, so
+ * define.h:13-15 supplies this truncated value) */
 #define FSX_PI_REFERENCE 3.14159265358
 
 typedef struct fsx_problem {

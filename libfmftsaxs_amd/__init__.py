@@ -238,7 +238,7 @@ def sbessel(l, x):
 
 
 def mkarray(begin, end, n):
-    """saxs_utils.c:50-63 q grid (cumulative addition, H3) from the library."""
+    """saxs_utils.c:50-63 q grid (cumulative addition) from the library."""
     L = lib()
     p = L.sxs_mkarray(begin, end, n)
     out = np.ctypeslib.as_array(p, shape=(n,)).copy()

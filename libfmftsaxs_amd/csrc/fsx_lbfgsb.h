@@ -20,8 +20,8 @@
  * visited points are the same).  Every arithmetic expression keeps the
  * reference's evaluation order; compile with FP contraction off so that the
  * iterates are bit-identical to the reference's for identical inputs.
- * The reference loop has no iteration cap (H9); FSX_LB_MAX_EVALS bounds it
- * here and the fit reports when the cap was hit.
+ * The reference loop has no iteration cap (min_saxs.c:234-248);
+ * FSX_LB_MAX_EVALS bounds it here.
  */
 #ifndef FSX_LBFGSB_H
 #define FSX_LBFGSB_H

@@ -261,7 +261,7 @@ void sxs_profile_from_spf(struct sxs_profile *p, struct sxs_spf_full *s, double 
     for (int q = 0; q < p->qnum; q++) {
         const double *vr = s->V[q]->re, *vi = s->V[q]->im, *dr = s->D[q]->re;
         const double *di = s->D[q]->im, *wr = s->W[q]->re, *wi = s->W[q]->im;
-        /* G at qvals[0] for every q, as in profile.c:214 (H8) */
+        /* G at qvals[0] for every q, as in profile.c:214 */
         double G = c1 * c1 * c1 * exp(corr * p->qvals[0] * p->qvals[0]);
         double in = 0.0;
         for (int l = 0; l <= s->L; l++)
@@ -285,7 +285,7 @@ double *scoring_helper(struct sxs_profile *exp, int qnum, double *qvals)
     int j = 0, nexp = exp->qnum;
     for (int i = 0; i < qnum; i++) {
         double lo = i > 0 ? qvals[i - 1] : qvals[0], hi = qvals[i];
-        /* the reference scans without a bound (H10); stop at the end */
+        /* the reference scans without a bound (min_saxs.c:369); stop at the end */
         while (j < nexp && exp->qvals[j] >= lo && exp->qvals[j] <= hi) {
             double e2 = exp->err[j] * exp->err[j];
             a[i * 6 + 0] += exp->in[j] * exp->in[j] / e2;

@@ -4,9 +4,9 @@
  * Wigner small-d table (borrowed.c:243-313) and the d-symbol table of the
  * translation operator (fftsaxs.c:138-174).
  *
- * The Bessel series and the Wigner recurrence are part of the parity
- * contract (SURVEY hazards H1/H2 pin their exact arithmetic), so their
- * evaluation order is kept alike the reference's; this file is built with
+ * The Bessel series (a truncated series whose large-argument values depend on
+ * its exact rounding) and the Wigner recurrence feed a 1e-10 parity bar, so
+ * their evaluation order is kept alike the reference's; this file is built with
  * FP contraction off.  The 3j symbols are computed here with binary128
  * arithmetic (Racah's sum) and truncated toward zero like the reference's
  * mpf_get_d; the table is built once per L per process.

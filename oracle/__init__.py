@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (FP64 CPU restatement of libfmftsaxs's FFT
 dimer SAXS scoring path).
 
-TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+TEST INFRASTRUCTURE: only tests/, the smoke entry point and bench.py's
 cpu_baseline leg may import this module, and only as the checker (or as the
 timed CPU baseline).  The shipped library never imports it.
 """
@@ -311,7 +311,7 @@ def direct_pose_cross_terms(A_re, A_im, B_re, B_im, L, qvals, z, b1, g1, a2, b2,
 
 
 def mkarray(begin, end, n):
-    """saxs_utils.c:50-63: q grid by cumulative addition (H3)."""
+    """saxs_utils.c:50-63: q grid by cumulative addition (q[i] = q[i-1] + step)."""
     step = (end - begin) / (n - 1)
     q = np.zeros(n)
     q[0] = begin

@@ -3,7 +3,7 @@
  * path, used ONLY as a test/benchmark checker.
  *
  * TEST INFRASTRUCTURE: nothing in the shipped library (libfmftsaxs_amd/)
- * links, imports or calls this code.  Only tests/, __graft_entry__.smoke()
+ * links, imports or calls this code.  Only tests/, the smoke entry point
  * and bench.py's cpu_baseline leg use it, as the checker or as the timed CPU
  * baseline.  Every function cites the reference file:line it restates.
  *
@@ -37,7 +37,8 @@ extern "C" {
 #define ORC_PI_EXACT 3.14159265358979323846
 
 /* Set/get the pi constant used by every oracle routine that the reference
- * computes with M_PI (H1). Default: ORC_PI_TRUNC. */
+ * computes with M_PI (the -std=c11 build takes define.h's truncated value).
+ * Default: ORC_PI_TRUNC. */
 void orc_set_pi(double pi);
 double orc_get_pi(void);
 
@@ -58,7 +59,7 @@ double orc_atoms2spf(int natoms, const double *coords, const double *vac,
                      const double *vdw, const double *qvals, int qnum, int L,
                      double *re, double *im);
 
-/* I(q) of one SPF at (c1, c2) (profile.c:186-233, G uses qvals[0]: H8). */
+/* I(q) of one SPF at (c1, c2) (profile.c:186-233, G uses qvals[0]: profile.c:214). */
 void orc_profile_from_spf(const double *re, const double *im, int L,
                           const double *qvals, int qnum, double rm,
                           double c1, double c2, double rerr,
@@ -79,7 +80,8 @@ typedef struct orc_fit_params {
 } orc_fit_params;
 
 /* scoring_helper, min_saxs.c:353-389.  n_exp bounds the experimental
- * arrays (H10: the reference reads past the end; here the scan stops). */
+ * arrays (the reference's loop has no bound and reads past the end; here the
+ * scan stops). */
 void orc_scoring_helper(const double *exp_q, const double *exp_in,
                         const double *exp_err, int n_exp,
                         const double *qvals, int qnum, double *a);

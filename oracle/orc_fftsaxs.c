@@ -9,7 +9,7 @@
  * TEST INFRASTRUCTURE (checker only; never linked into the product).
  *
  * Differences from the reference that do not change any computed value:
- * poses are bucketed by slab once instead of rescanned per slab (H13),
+ * poses are bucketed by slab once instead of rescanned per slab,
  * only requested grid points are materialised, slabs of one z run in
  * parallel (OpenMP), and the dense-slab transform (FFTW in the reference,
  * fftsaxs.c:899-900) is an exact-twiddle direct DFT at the requested points

@@ -18,7 +18,8 @@ enum { VV = 0, VD = 1, VW = 2, DD = 3, DW = 4, WW = 5 };
 
 /* min_saxs.c:353-389.  Moments of the experimental curve per model q bin
  * [q_{i-1}, q_i] (inclusive), normalised by the number of consumed points.
- * The scan stops at n_exp (the reference has no bound: H10). */
+ * The scan stops at n_exp (the reference's while loop has no bound and reads
+ * past the experimental arrays when they end inside the last bin). */
 void orc_scoring_helper(const double *eq, const double *ein, const double *eerr,
                         int n_exp, const double *qvals, int qnum, double *a)
 {
@@ -288,7 +289,7 @@ double orc_atoms2spf(int natoms, const double *xyz, const double *vac,
     return rm;
 }
 
-/* profile.c:186-233; note G uses qvals[0] for every q (H8). */
+/* profile.c:186-233; note G uses qvals[0] for every q (profile.c:214). */
 void orc_profile_from_spf(const double *re, const double *im, int L,
                           const double *qvals, int qnum, double rm,
                           double c1, double c2, double rerr,

@@ -3,7 +3,7 @@ restatement of the reference).  Bars, stated per check:
   * Bessel table, fit of identical profiles: bit-exact except where the
     device and host libm (pow, exp) round differently (counted, <= 2 ulp);
   * cross-term profiles / I(q): within 1e-10 relative of the oracle
-    (north star: I(q) within 1e-10 relative);
+    (the parity bar of the FP64 form);
   * score, c1, c2: within 1e-7 relative / 1e-6 absolute and identical
     score ranking of the top poses.
 """

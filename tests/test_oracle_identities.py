@@ -41,7 +41,7 @@ def test_sbessel_against_closed_forms(orc):
         j0 = np.sin(x) / x
         j1 = np.sin(x) / x ** 2 - np.cos(x) / x
         j2 = (3 / x ** 2 - 1) * np.sin(x) / x - 3 * np.cos(x) / x ** 2
-        # the reference's series stops at a 1e-5 relative term (H2)
+        # the reference's series stops at a 1e-5 relative term (sfbessel.c:46)
         assert abs(orc.sbessel(0, x) - j0) <= 2e-5 * max(abs(j0), 1e-3)
         assert abs(orc.sbessel(1, x) - j1) <= 2e-5 * max(abs(j1), 1e-3)
         assert abs(orc.sbessel(2, x) - j2) <= 2e-5 * max(abs(j2), 1e-3)
@@ -94,7 +94,7 @@ def test_slab_dft_equals_direct_pose(orc, prob):
 
 def test_truncated_twiddles_close_to_exact(orc, prob):
     """fftsaxs.c:899: slabs with < 30 poses use truncated-pi twiddles; the
-    effect on I(q) is ~1e-12 relative (H4) and both paths are exercised."""
+    effect on I(q) is ~1e-12 relative and both paths are exercised."""
     pb = prob
     L = pb["L"]
     kw = dict(want_cross=True)
@@ -188,7 +188,7 @@ def test_translation_along_axis_matches_moved_atoms(orc):
     """A ligand translated by +z along the axis (pose z, all angles 0): the
     cross terms from the translation operator agree with the expansion of
     the physically moved dimer at low q, where the L = 15 expansion has
-    converged (limited by the reference's 1e-5 Bessel series, H2)."""
+    converged (limited by the reference's 1e-5 Bessel series)."""
     from libfmftsaxs_amd import synthetic as S
     rec = S.random_chain(60, 11)
     lig = S.random_chain(40, 12)
@@ -215,12 +215,12 @@ def test_index_round_trip_and_carry(orc):
     for t in [(0, 0, 0, 0, 0, 0), (3, 7, 15, 30, 12, 1), (79, 15, 15, 30, 30, 30)]:
         i = orc.assemble_index(*t, L + 1, N, N)
         assert orc.disassemble_index(i, L + 1, N, N) == t
-    # H5: a2 = 2 pi - 0.0005 rounds to digit N and carries into b2
+    # a2 = 2 pi - 0.0005 rounds to digit N, which carries into b2 (tools/correlate.c:222-238)
     zv = np.arange(1.0, 81.0)
     pi = orc.PI_TRUNC
     (idx,) = orc.euler2index(40.0, 0.0, 0.0, 0.0005, 0.0, 0.0, zv, L)
     z, b1, b2, a2, g1, g2 = orc.disassemble_index(idx, L + 1, N, N)
     assert (z, b1, a2) == (39, 0, 0) and b2 == 1
-    # a pose whose z is not within 0.001 of a grid z is dropped (H6)
+    # a pose whose z is not within 0.001 of a grid z is dropped (tools/correlate.c:219-220)
     assert orc.euler2index(40.002, 1.0, 1.0, 1.0, 1.0, 1.0, zv, L) == []
     assert pi == 3.14159265358
