@@ -54,40 +54,65 @@ def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60
     log(f"workload built in {time.time() - t0:.1f} s")
     return dict(problem=pb, poses=poses, q=q, zv=zv, L=L, A=(Ar, Ai), B=(Br, Bi), rm=rm,
                 a=a, mult=mult, peak=peak, exp=(q, Iexp, 0.05 * Iexp), nbeta=nbeta,
-                ngrid=ngrid)
+                ngrid=ngrid, c1=c1, c2=c2)
 
 
 def cpu_baseline(w, max_threads=16):
     """The oracle (FP64 CPU restatement of the reference algorithm: per-slab
-    second sums, DFT to the Euler grid, one L-BFGS-B fit per pose) on a
-    bounded sample: the same molecules and q grid on the reference's own
-    native grid (L+1 betas, 2L+1-point alpha/gamma), the poses of one z plane
-    (z = 40 A) with receptor beta index < 4, from a 210,000-pose native-grid
-    set."""
+    second sums, transform to the requested grid points, one L-BFGS-B fit
+    per pose, OpenMP over slabs) on a bounded sample of this workload: the
+    poses of the z = 40 A plane with receptor beta index < 6."""
     import oracle as O
     O.set_pi(O.PI_TRUNC)
-    L = w["L"]
-    N = 2 * L + 1
-    from libfmftsaxs_amd import synthetic as S
-    zv = w["zv"]
-    idx = S.random_grid_poses(210000, zv.size, L + 1, N, N, 5001)
-    size5d = (L + 1) ** 2 * N ** 3
-    zsel = int(np.argmin(np.abs(zv - 40.0)))
-    b1 = (idx % size5d) // ((L + 1) * N ** 3)
-    sample = idx[(idx // size5d == zsel) & (b1 < 4)]
-    P = O.Params(w["a"], w["rm"], w["peak"])
+    L, nb, ng = w["L"], w["nbeta"], w["ngrid"]
+    poses = w["poses"]
+    size5d = nb * nb * ng ** 3
+    zsel = int(np.argmin(np.abs(w["zv"] - 40.0)))
+    b1 = (poses % size5d) // (nb * ng ** 3)
+    sample = poses[(poses // size5d == zsel) & (b1 < 6)]
+    P = O.Params(w["a"], w["rm"], w["peak"], c1=w["c1"], c2=w["c2"])
     P.s.mult = w["mult"]
     nthr = max(1, min(max_threads, os.cpu_count() or 1))
     t = time.time()
     r = O.compute_saxs_scores(sample, w["A"][0], w["A"][1], w["B"][0], w["B"][1], P, w["q"],
-                              zv, L, nthreads=nthr)
+                              w["zv"], L, nbeta=nb, na=ng, ng=ng, nthreads=nthr)
     dt = time.time() - t
     assert not np.any(np.isnan(r["score"]))
     return {"value": len(sample) / dt, "unit": "poses/s", "cores": nthr, "kind": "port",
-            "sample": f"{len(sample)} poses: those of one z plane (z=40 A) with beta_rec index < 4 "
-                      f"from a 210000-pose "
-                      f"native-grid set (L=15, 16 betas, 31-point alpha/gamma), same "
-                      f"molecules and 51 q; oracle OpenMP over slabs, {dt:.1f} s"}
+            "sample": f"{len(sample)} poses of this workload (the z = 40 A plane, receptor "
+                      f"beta index < 6; {6 * nb} slabs), oracle with OpenMP over slabs on "
+                      f"{nthr} threads, {dt:.1f} s"}
+
+
+def w_alg(poses, L, Q, nb, ng):
+    """W_alg of the reference's slab algorithm for these poses (compute_sum1
+    per active (z, beta2), compute_sum2 per active (z, beta1, beta2) slab;
+    fftsaxs.c:768-799, 416-524)."""
+    C1 = sum(1 for m in range(-L, L + 1) for m2 in range(-L, L + 1)
+             for l in range(abs(m), L + 1) for l1 in range(max(abs(m), abs(m2)), L + 1))
+    C2 = sum(L + 1 - max(abs(m), abs(m1)) for m in range(-L, L + 1)
+             for m1 in range(-L, L + 1)) * (2 * L + 1)
+    size5d = nb * nb * ng ** 3
+    z = poses // size5d
+    r = poses % size5d
+    b1 = r // (nb * ng ** 3)
+    b2 = (r // ng ** 3) % nb
+    zb2 = np.unique(z * nb + b2).size
+    slabs = np.unique((z * nb + b1) * nb + b2).size
+    return 24.0 * Q * C1 * zb2 + 72.0 * Q * C2 * slabs
+
+
+def load_traffic():
+    """HBM bytes per launch of the main kernels, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by what they
+    read for a plain copy of known size); {} if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return {k: v["bytes_per_launch"] for k, v in d["kernels"].items()}
+    except (OSError, KeyError, ValueError):
+        return {}
 
 
 def main():
@@ -97,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--poses", type=int, default=210000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times (a plain copy "
+                         "kernel of known size, to calibrate HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build targets single-GPU boxes "
@@ -104,6 +132,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "reads_per_copy": nb, "writes_per_copy": nb}))
+        return
     import libfmftsaxs_amd as F
     if F.device_count() < 1:
         raise SystemExit("no HIP device")
@@ -138,22 +176,33 @@ def main():
     avg = {k: float(np.mean([s[k] for s in stats])) for k in keys}
     log("per-stage ms: " + ", ".join(f"{k[3:]}={v:.3f}" for k, v in avg.items()))
     cb = float(np.mean([s["cross_bytes"] for s in stats]))
-    # dominant kernel and its roofline
+    cf = float(np.mean([s["cross_flops"] for s in stats]))
+    # dominant kernel and its roofline.  The cross-term kernel is FP64
+    # arithmetic on operands held in LDS and registers: its roof is the FP64
+    # peak (78.6 TFLOP/s; on MI355X the FP64 vector and FP64 MFMA peaks are
+    # the same).  The fit stream is priced against HBM (its profile read once
+    # and three results written per pose).
+    traffic = load_traffic()
     if avg["ms_cross"] >= avg["ms_fit"]:
-        dom = "k_cross"
         t_k = avg["ms_cross"] * 1e-3
-        achieved = cb / t_k / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
-                "ms_per_launch": avg["ms_cross"], "algorithmic_bytes": cb}
+        achieved = cf / t_k / 1e12
+        roof = {"kernel": "k_cross", "bound": "mfma", "achieved": achieved, "peak": 78.6,
+                "unit": "TFLOP/s", "frac": achieved / 78.6,
+                "traffic": traffic.get("k_cross"), "ms_per_launch": avg["ms_cross"],
+                "algorithmic_flops": cf, "algorithmic_bytes": cb}
     else:
-        dom = "k_fit"
         t_k = avg["ms_fit"] * 1e-3
-        fb = n * (6 * pb.qnum * 8 + 24 + 4)
+        fb = n * (6 * pb.qnum * 8 + 24)
         achieved = fb / t_k / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
+        roof = {"kernel": "k_fit", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                "unit": "GB/s", "frac": achieved / 8000.0, "traffic": traffic.get("k_fit"),
                 "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb}
+    # the reference algorithm's work for the same poses (per active (z, beta2)
+    # 24 Q C1(L) flops of first sums, per active (z, beta1, beta2) slab
+    # 72 Q C2(L) flops of second sums), as an effective rate
+    walg = w_alg(w["poses"], w["L"], int(pb.qnum), w["nbeta"], w["ngrid"])
+    roof["w_alg_flops"] = walg
+    roof["w_alg_effective_tflops"] = walg / (ms * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -164,7 +213,7 @@ def main():
            "value": value, "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "config 1: receptor 3000 / ligand 1500 atoms, L=15, 51 q "
+           "config": {"workload": "receptor 3000 / ligand 1500 atoms, L=15, 51 q "
                                   "on [0,0.5], z 25..60 step 1, 19 betas (10 deg), 64-point "
                                   "alpha/gamma grids, 210000 poses, c1 [0.95,1.05], "
                                   "c2 [0,4], FP64",

@@ -76,8 +76,11 @@ typedef struct fsx_stats {
     double ms_cross;     /* per-pose cross-term profiles (GPU) */
     double ms_fit;       /* per-pose chi^2 fits (GPU) */
     double ms_total;     /* whole call, wall clock */
-    double cross_bytes;  /* algorithmic bytes moved by the cross-term kernel */
-    double cross_flops;  /* executed flops of the cross-term kernel */
+    double cross_bytes;  /* algorithmic bytes of the cross-term launches (receptor rows,
+                            ligand rows, per-pose accumulators) */
+    double cross_flops;  /* useful FP64 flops of the cross-term launches */
+    long cross_items;    /* non-empty (z, beta2, gamma2) buckets x q */
+    int cross_chunks;    /* m-chunks (cross-term launches) used */
 } fsx_stats;
 
 /* Score n poses.  score/c1/c2 receive one value per pose whose z slot is

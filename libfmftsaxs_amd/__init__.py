@@ -38,7 +38,8 @@ class Stats(C.Structure):
                 ("ms_tables", C.c_double), ("ms_bessel_t", C.c_double),
                 ("ms_rotate", C.c_double), ("ms_group", C.c_double),
                 ("ms_cross", C.c_double), ("ms_fit", C.c_double), ("ms_total", C.c_double),
-                ("cross_bytes", C.c_double), ("cross_flops", C.c_double)]
+                ("cross_bytes", C.c_double), ("cross_flops", C.c_double),
+                ("cross_items", C.c_long), ("cross_chunks", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -86,26 +86,52 @@ typedef struct fsx_fitpar {
     double factr, pgtol;
 } fsx_fitpar;
 
+/* The six terms at q index i, each (c + v) * scale as fsx_ct_get.  Fetched
+ * together (and one q ahead in the loops below) so that a GPU thread has
+ * all its loads of a step in flight at once. */
+FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
+{
+    const double *c = t->c + i * t->cqs;
+    if (t->v) {
+        const double *v = t->v + i * t->vqs;
+        for (int k = 0; k < 6; k++) {
+            double x = c[k * t->cks];
+            x = x + v[k * t->vks];
+            o[k] = x * t->scale;
+        }
+    } else {
+        for (int k = 0; k < 6; k++) o[k] = c[k * t->cks] * t->scale;
+    }
+}
+
+/* I(q) at G = c1^3 exp(-mult (c1^2 - 1) q^2) and c2 (min_saxs.c:285-287) */
+#define FSX_IN(v, G, c2)                                                              \
+    ((v)[0] - (G) * (v)[1] + (c2) * (v)[2] + (G) * (G) * (v)[3] - (G) * (c2) * (v)[4] \
+     + (c2) * (c2) * (v)[5])
+
 FSX_HD double fsx_in(const fsx_ct *t, int i, double G, double c2)
 {
-    return fsx_ct_get(t, 0, i) - G * fsx_ct_get(t, 1, i) + c2 * fsx_ct_get(t, 2, i)
-         + G * G * fsx_ct_get(t, 3, i) - G * c2 * fsx_ct_get(t, 4, i)
-         + c2 * c2 * fsx_ct_get(t, 5, i);
+    double v[6];
+    fsx_ct_get6(t, i, v);
+    return FSX_IN(v, G, c2);
 }
 
 /* min_saxs.c:261-319 */
 FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2)
 {
     const double *a = p->a, *q = p->q;
+    double cur[6], nxt[6];
+    fsx_ct_get6(t, 0, cur);
     double corr = -p->mult * (c1 * c1 - 1.0);
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
-    double in_prev = fsx_in(t, 0, G, c2);
+    double in_prev = FSX_IN(cur, G, c2);
     double c1c = c1 * c1 * c1;
     double q_prev = -1.0, up = 0.0, down = 0.0;
     for (int i = 0; i < p->qnum; i++) {
+        if (i + 1 < p->qnum) fsx_ct_get6(t, i + 1, nxt);
         double qc = q[i];
         G = c1c * exp(corr * qc * qc);
-        double in = fsx_in(t, i, G, c2);
+        double in = FSX_IN(cur, G, c2);
         double tan_ = (in - in_prev) / (qc - q_prev);
         double buf = in - tan_ * qc;
         up += buf * a[i * 6 + 1] + tan_ * a[i * 6 + 2];
@@ -113,6 +139,8 @@ FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, do
               + tan_ * tan_ * a[i * 6 + 5];
         in_prev = in;
         q_prev = qc;
+        if (i + 1 < p->qnum)
+            for (int k = 0; k < 6; k++) cur[k] = nxt[k];
     }
     return up / down;
 }
@@ -122,23 +150,25 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
                           const double *x, double *g, double *fval)
 {
     const double *a = p->a, *q = p->q;
+    double cur[6], nxt[6];
+    fsx_ct_get6(t, 0, cur);
     double c1 = x[0], c2 = x[1];
     double g0 = 0.0, g1 = 0.0, mult = p->mult;
     double corr = -mult * (c1 * c1 - 1.0);
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
     double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * q[0] * q[0]);
-    double in_prev = fsx_in(t, 0, G, c2);
-    double d1p = -Gd * fsx_ct_get(t, 1, 0) + 2.0 * G * Gd * fsx_ct_get(t, 3, 0)
-               - Gd * c2 * fsx_ct_get(t, 4, 0);
-    double d2p = fsx_ct_get(t, 2, 0) - G * fsx_ct_get(t, 4, 0) + 2.0 * c2 * fsx_ct_get(t, 5, 0);
+    double in_prev = FSX_IN(cur, G, c2);
+    double d1p = -Gd * cur[1] + 2.0 * G * Gd * cur[3] - Gd * c2 * cur[4];
+    double d2p = cur[2] - G * cur[4] + 2.0 * c2 * cur[5];
     double q_prev = -1.0, score = 0.0, c1c = c1 * c1 * c1;
     for (int i = 0; i < p->qnum; i++) {
+        if (i + 1 < p->qnum) fsx_ct_get6(t, i + 1, nxt);
         double qc = q[i];
         G = c1c * exp(corr * qc * qc);
         Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
-        double vd = fsx_ct_get(t, 1, i), vw = fsx_ct_get(t, 2, i), dd = fsx_ct_get(t, 3, i);
-        double dw = fsx_ct_get(t, 4, i), ww = fsx_ct_get(t, 5, i);
-        double in = fsx_ct_get(t, 0, i) - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww;
+        double vd = cur[1], vw = cur[2], dd = cur[3];
+        double dw = cur[4], ww = cur[5];
+        double in = cur[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww;
         double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);
         double d2 = vw - G * dw + 2.0 * c2 * ww;
         double buf = 1.0 / (qc - q_prev);
@@ -162,6 +192,8 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
         d1p = d1;
         d2p = d2;
         q_prev = qc;
+        if (i + 1 < p->qnum)
+            for (int j = 0; j < 6; j++) cur[j] = nxt[j];
     }
     g[0] = g0;
     g[1] = g1;
@@ -956,39 +988,73 @@ FSX_HD void lb_eval(const fsx_ct *t, const fsx_fitpar *p, const double *x, doubl
     fsx_objective(t, p, k, x, g, f);
 }
 
-/* One complete fit.  Returns the final task; x = (c1, c2), *f = chi^2,
- * *nevals = objective evaluations.  Bounds are both-sided (nbd = 2). */
-FSX_HD int fsx_lbfgsb_fit(const fsx_ct *ct, const fsx_fitpar *p, double *x, double *fout,
-                          int *nevals)
-{
-    fsx_lb W;
-    fsx_lb *w = &W;
-    const int n = FSX_LB_N;
-    lb_zero(w);
-    const double l[2] = {p->lo0, p->lo1}, u[2] = {p->hi0, p->hi1};
-    const double epsmch = DBL_EPSILON;
-    int col = 0, head = 1, iupdat = 0, updatd = 0, iback = 0, itail = 0;
-    int iter = 0, nseg = 0, nfree = n, ifun = 0, info = 0, nenter = 0, ileave = 0;
-    double theta = 1., fold = 0., dnorm = 0., gd = 0., stpmx = 0.;
-    double sbgnrm, stp = 0., gdold = 0., dtd = 0., f = 0., kscale = 1.0;
-    double tol = p->factr * epsmch;
-    double g[2] = {0.0, 0.0};
-    int wrk = 0, task, csave = 0, nev = 0;
+/* ---------------------------------------------------------------------- */
+/* Resumable driver.  setulb's reverse communication (lbfgsb.c:52-515, driven
+ * by min_saxs.c:234-248) is kept as an explicit state machine:
+ * fsx_lb_start() sets the first trial point s->x; the caller evaluates f
+ * and g there into s->f / s->g and calls fsx_lb_resume(), which runs the
+ * optimiser up to the next point it needs evaluated (returns 1, point in
+ * s->x) or to the end of the fit (returns 0, reason in s->task).  Since
+ * every NEW_X is resumed at once, the visited points are those of the
+ * reference.  Many fits can share one evaluation loop this way, and the
+ * slot of a finished fit can start the next one. */
+#define FSX_PH_INIT 0
+#define FSX_PH_LS 1
+#define FSX_PH_DONE 2
+
+typedef struct fsx_lbrc {
+    fsx_lb w;
     fsx_ls ls;
-    ls.brackt = 0; ls.stage = 0;
-    ls.ginit = ls.gtest = ls.gx = ls.gy = ls.finit = ls.fx = ls.fy = 0.0;
-    ls.stx = ls.sty = ls.stmin = ls.stmax = ls.width = ls.width1 = 0.0;
-    /* active: project x0, classify (all variables boxed) */
-    for (int i = 0; i < n; i++) {
-        if (x[i] <= l[i]) {
-            if (x[i] < l[i]) x[i] = l[i];
-        } else if (x[i] >= u[i]) {
-            if (x[i] > u[i]) x[i] = u[i];
+    double x[FSX_LB_N], g[FSX_LB_N], f;
+    double l[FSX_LB_N], u[FSX_LB_N];
+    double theta, fold, gd, stpmx, sbgnrm, stp, gdold, dtd, tol, pgtol;
+    int col, head, iupdat, updatd, iback, itail, iter, nseg, nfree, ifun, nenter, ileave;
+    int wrk, csave, nev, task, phase;
+} fsx_lbrc;
+
+/* x0 = (c1, c2) projected into the box [lo, hi] (active, lbfgsb.c:540-600,
+ * every variable bounded on both sides); the first evaluation is due at
+ * s->x. */
+FSX_HD void fsx_lb_start(fsx_lbrc *s, const fsx_fitpar *p, double x0, double x1)
+{
+    lb_zero(&s->w);
+    s->ls.brackt = 0; s->ls.stage = 0;
+    s->ls.ginit = s->ls.gtest = s->ls.gx = s->ls.gy = s->ls.finit = s->ls.fx = s->ls.fy = 0.0;
+    s->ls.stx = s->ls.sty = s->ls.stmin = s->ls.stmax = s->ls.width = s->ls.width1 = 0.0;
+    s->l[0] = p->lo0; s->l[1] = p->lo1;
+    s->u[0] = p->hi0; s->u[1] = p->hi1;
+    s->x[0] = x0; s->x[1] = x1;
+    s->g[0] = 0.0; s->g[1] = 0.0; s->f = 0.0;
+    s->col = 0; s->head = 1; s->iupdat = 0; s->updatd = 0; s->iback = 0; s->itail = 0;
+    s->iter = 0; s->nseg = 0; s->nfree = FSX_LB_N; s->ifun = 0; s->nenter = 0; s->ileave = 0;
+    s->theta = 1.; s->fold = 0.; s->gd = 0.; s->stpmx = 0.; s->sbgnrm = 0.; s->stp = 0.;
+    s->gdold = 0.; s->dtd = 0.;
+    s->tol = p->factr * DBL_EPSILON;
+    s->pgtol = p->pgtol;
+    s->wrk = 0; s->csave = 0; s->nev = 0; s->task = FSX_T_FG; s->phase = FSX_PH_INIT;
+    for (int i = 0; i < FSX_LB_N; i++) {
+        if (s->x[i] <= s->l[i]) {
+            if (s->x[i] < s->l[i]) s->x[i] = s->l[i];
+        } else if (s->x[i] >= s->u[i]) {
+            if (s->x[i] > s->u[i]) s->x[i] = s->u[i];
         }
-        w->iwhere[i] = (u[i] - l[i] <= 0.) ? 3 : 0;
+        s->w.iwhere[i] = (s->u[i] - s->l[i] <= 0.) ? 3 : 0;
     }
-    lb_eval(ct, p, x, &f, g, &kscale);
-    nev++;
+}
+
+/* mainlb from the point after an evaluation to the next one (or the end). */
+FSX_HD int fsx_lb_resume(fsx_lbrc *s)
+{
+    fsx_lb *w = &s->w;
+    const int n = FSX_LB_N;
+    double *x = s->x, *g = s->g;
+    const double *l = s->l, *u = s->u;
+    const double epsmch = DBL_EPSILON;
+    int info = 0, restart, skip_search;
+    double rr, dr, ddum;
+    if (s->phase == FSX_PH_DONE) return 0;
+    s->nev++;
+    if (s->phase == FSX_PH_LS) goto ls_resume;
 #define LB_PROJGR(out)                                                          \
     do {                                                                        \
         double s_ = 0.;                                                         \
@@ -1000,133 +1066,155 @@ FSX_HD int fsx_lbfgsb_fit(const fsx_ct *ct, const fsx_fitpar *p, double *x, doub
         }                                                                       \
         (out) = s_;                                                             \
     } while (0)
-    LB_PROJGR(sbgnrm);
-    if (sbgnrm <= p->pgtol) { task = FSX_T_CONV_GRAD; goto done; }
+#define LB_RESET()                                                              \
+    do {                                                                        \
+        info = 0; s->col = 0; s->head = 1; s->theta = 1.;                       \
+        s->iupdat = 0; s->updatd = 0;                                           \
+    } while (0)
+    /* the first evaluation (at the projected x0) is done */
+    LB_PROJGR(s->sbgnrm);
+    if (s->sbgnrm <= s->pgtol) { s->task = FSX_T_CONV_GRAD; goto done; }
     for (;;) {
-        info = lb_cauchy(w, x, l, u, g, theta, col, head, sbgnrm, &nseg);
+        info = lb_cauchy(w, x, l, u, g, s->theta, s->col, s->head, s->sbgnrm, &s->nseg);
         if (info) {
-            info = 0; col = 0; head = 1; theta = 1.; iupdat = 0; updatd = 0;
+            LB_RESET();
             continue;
         }
-        lb_freev(w, &nfree, &nenter, &ileave, &wrk, updatd, iter);
-        if (!(nfree == 0 || col == 0)) {
-            if (wrk) info = lb_formk(w, nfree, nenter, ileave, iupdat, updatd, theta, col, head);
+        lb_freev(w, &s->nfree, &s->nenter, &s->ileave, &s->wrk, s->updatd, s->iter);
+        if (!(s->nfree == 0 || s->col == 0)) {
+            if (s->wrk)
+                info = lb_formk(w, s->nfree, s->nenter, s->ileave, s->iupdat, s->updatd,
+                                s->theta, s->col, s->head);
             if (info) {
-                info = 0; col = 0; head = 1; theta = 1.; iupdat = 0; updatd = 0;
+                LB_RESET();
                 continue;
             }
-            info = lb_cmprlb(w, x, g, theta, col, head, nfree);
-            if (!info) info = lb_subsm(w, nfree, l, u, x, g, theta, col, head);
+            info = lb_cmprlb(w, x, g, s->theta, s->col, s->head, s->nfree);
+            if (!info) info = lb_subsm(w, s->nfree, l, u, x, g, s->theta, s->col, s->head);
             if (info) {
-                info = 0; col = 0; head = 1; theta = 1.; iupdat = 0; updatd = 0;
+                LB_RESET();
                 continue;
             }
         }
         for (int i = 0; i < n; i++) w->d[i] = w->z[i] - x[i];
-        /* lnsrlb entry (constrained, boxed) */
-        dtd = lb_dot(n, w->d, w->d);
-        dnorm = sqrt(dtd);
-        (void)dnorm;
-        if (iter == 0) {
-            stpmx = 1.;
+        /* lnsrlb entry (every variable boxed) */
+        s->dtd = lb_dot(n, w->d, w->d);
+        if (s->iter == 0) {
+            s->stpmx = 1.;
         } else {
-            stpmx = 1e10;
+            s->stpmx = 1e10;
             for (int i = 0; i < n; i++) {
                 double a1 = w->d[i];
                 if (a1 < 0.) {
                     double a2 = l[i] - x[i];
-                    if (a2 >= 0.) stpmx = 0.;
-                    else if (a1 * stpmx < a2) stpmx = a2 / a1;
+                    if (a2 >= 0.) s->stpmx = 0.;
+                    else if (a1 * s->stpmx < a2) s->stpmx = a2 / a1;
                 } else if (a1 > 0.) {
                     double a2 = u[i] - x[i];
-                    if (a2 <= 0.) stpmx = 0.;
-                    else if (a1 * stpmx > a2) stpmx = a2 / a1;
+                    if (a2 <= 0.) s->stpmx = 0.;
+                    else if (a1 * s->stpmx > a2) s->stpmx = a2 / a1;
                 }
             }
         }
-        stp = 1.;
+        s->stp = 1.;
         for (int i = 0; i < n; i++) { w->t[i] = x[i]; w->r[i] = g[i]; }
-        fold = f;
-        ifun = 0;
-        iback = 0;
-        csave = FSX_T_START;
-        int restart = 0;
+        s->fold = s->f;
+        s->ifun = 0;
+        s->iback = 0;
+        s->csave = FSX_T_START;
+        restart = 0;
         for (;;) {
-            gd = lb_dot(n, g, w->d);
-            int skip_search = 0;
-            if (ifun == 0) {
-                gdold = gd;
-                if (gd >= 0.) { info = -4; skip_search = 1; }
+            s->gd = lb_dot(n, g, w->d);
+            skip_search = 0;
+            if (s->ifun == 0) {
+                s->gdold = s->gd;
+                if (s->gd >= 0.) { info = -4; skip_search = 1; }
             }
             if (!skip_search) {
-                lb_dcsrch(f, gd, &stp, 0., stpmx, &csave, &ls);
-                if (!(csave >= 100 && csave <= 110) && !(csave >= 20 && csave <= 25)) {
-                    task = FSX_T_FG_LN;
-                    ++ifun;
-                    iback = ifun - 1;
-                    if (stp == 1.) {
+                lb_dcsrch(s->f, s->gd, &s->stp, 0., s->stpmx, &s->csave, &s->ls);
+                if (!(s->csave >= 100 && s->csave <= 110) && !(s->csave >= 20 && s->csave <= 25)) {
+                    s->task = FSX_T_FG_LN;
+                    ++s->ifun;
+                    s->iback = s->ifun - 1;
+                    if (s->stp == 1.) {
                         for (int i = 0; i < n; i++) x[i] = w->z[i];
                     } else {
-                        for (int i = 0; i < n; i++) x[i] = stp * w->d[i] + w->t[i];
+                        for (int i = 0; i < n; i++) x[i] = s->stp * w->d[i] + w->t[i];
                     }
                 } else {
-                    task = FSX_T_NEW_X;
+                    s->task = FSX_T_NEW_X;
                 }
             }
-            if (info != 0 || iback >= 20) {
+            if (info != 0 || s->iback >= 20) {
                 for (int i = 0; i < n; i++) { x[i] = w->t[i]; g[i] = w->r[i]; }
-                f = fold;
-                if (col == 0) {
-                    task = FSX_T_ABNORMAL;
-                    ++iter;
+                s->f = s->fold;
+                if (s->col == 0) {
+                    s->task = FSX_T_ABNORMAL;
+                    ++s->iter;
                     goto done;
                 }
-                info = 0; col = 0; head = 1; theta = 1.; iupdat = 0; updatd = 0;
+                LB_RESET();
                 restart = 1;
                 break;
-            } else if (task == FSX_T_FG_LN) {
-                if (nev >= FSX_LB_MAX_EVALS) { task = FSX_T_STOP_ITER; goto done; }
-                lb_eval(ct, p, x, &f, g, &kscale);
-                nev++;
+            } else if (s->task == FSX_T_FG_LN) {
+                if (s->nev >= FSX_LB_MAX_EVALS) { s->task = FSX_T_STOP_ITER; goto done; }
+                s->phase = FSX_PH_LS;
+                return 1;
+            ls_resume:
                 continue;
             } else {
-                ++iter;
-                LB_PROJGR(sbgnrm);
+                ++s->iter;
+                LB_PROJGR(s->sbgnrm);
                 break;
             }
         }
         if (restart) continue;
-        if (sbgnrm <= p->pgtol) { task = FSX_T_CONV_GRAD; goto done; }
-        {
-            double ddum = lb_max(lb_max(lb_abs(fold), lb_abs(f)), 1.);
-            if (fold - f <= tol * ddum) { task = FSX_T_CONV_F; goto done; }
-        }
+        if (s->sbgnrm <= s->pgtol) { s->task = FSX_T_CONV_GRAD; goto done; }
+        ddum = lb_max(lb_max(lb_abs(s->fold), lb_abs(s->f)), 1.);
+        if (s->fold - s->f <= s->tol * ddum) { s->task = FSX_T_CONV_F; goto done; }
         for (int i = 0; i < n; i++) w->r[i] = g[i] - w->r[i];
-        double rr = lb_dot(n, w->r, w->r), dr, ddum;
-        if (stp == 1.) {
-            dr = gd - gdold;
-            ddum = -gdold;
+        rr = lb_dot(n, w->r, w->r);
+        if (s->stp == 1.) {
+            dr = s->gd - s->gdold;
+            ddum = -s->gdold;
         } else {
-            dr = (gd - gdold) * stp;
-            for (int i = 0; i < n; i++) w->d[i] = stp * w->d[i];
-            ddum = -gdold * stp;
+            dr = (s->gd - s->gdold) * s->stp;
+            for (int i = 0; i < n; i++) w->d[i] = s->stp * w->d[i];
+            ddum = -s->gdold * s->stp;
         }
         if (dr <= epsmch * ddum) {
-            updatd = 0;
+            s->updatd = 0;
             continue;
         }
-        updatd = 1;
-        ++iupdat;
-        lb_matupd(w, &itail, iupdat, &col, &head, &theta, rr, dr, stp, dtd);
-        if (lb_formt(w, col, theta)) {
-            info = 0; col = 0; head = 1; theta = 1.; iupdat = 0; updatd = 0;
-        }
+        s->updatd = 1;
+        ++s->iupdat;
+        lb_matupd(w, &s->itail, s->iupdat, &s->col, &s->head, &s->theta, rr, dr, s->stp,
+                  s->dtd);
+        if (lb_formt(w, s->col, s->theta)) LB_RESET();
     }
 #undef LB_PROJGR
+#undef LB_RESET
 done:
-    *fout = f;
-    *nevals = nev;
-    return task;
+    s->phase = FSX_PH_DONE;
+    return 0;
+}
+
+/* One complete fit from x = (c1, c2).  Returns the final task; x is the
+ * result, *fout = chi^2, *nevals = objective evaluations. */
+FSX_HD int fsx_lbfgsb_fit(const fsx_ct *ct, const fsx_fitpar *p, double *x, double *fout,
+                          int *nevals)
+{
+    fsx_lbrc S;
+    double k;
+    fsx_lb_start(&S, p, x[0], x[1]);
+    do {
+        lb_eval(ct, p, S.x, &S.f, S.g, &k);
+    } while (fsx_lb_resume(&S));
+    x[0] = S.x[0];
+    x[1] = S.x[1];
+    *fout = S.f;
+    *nevals = S.nev;
+    return S.task;
 }
 
 #endif

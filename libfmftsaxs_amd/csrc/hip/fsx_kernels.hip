@@ -118,8 +118,20 @@ __global__ void k_sbessel_batch(const double *x, int n, int P, double *out)
     out[i] = dev_sbessel((int)(i % P), x[i / P]);
 }
 
-/* T[z][q][m][l][l1], m >= 0, computed from (min(l,l1), max(l,l1)) so the
- * mirrored entries are bitwise equal, as in fftsaxs.c:213-238. */
+/* Packed translation blocks: per (z, q) the rows (|m|, l) for l >= |m| in
+ * order of |m| then l, each holding l1 = |m|..L.  Row (am, l) starts at
+ * S(K) - S(K - am) + (l - am)(K - am), S(n) = n(n+1)(2n+1)/6; the block has
+ * S(K) entries. */
+__host__ __device__ __forceinline__ int tpk_sq(int n) { return n * (n + 1) * (2 * n + 1) / 6; }
+__host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
+{
+    return tpk_sq(K) - tpk_sq(K - am) + (l - am) * (K - am);
+}
+
+/* T^{|m|}_{l l1}(q z) (fftsaxs.c:182-243), computed from (min(l,l1), max(l,l1))
+ * so the mirrored entries are bitwise equal, as in fftsaxs.c:213-238.  One
+ * thread per (z, q, am, l, l1) of the full cube; valid entries are stored in
+ * the packed layout. */
 __global__ void k_tmat(cplx *T, const double *bes, const double *ds, int znum, int Q, int L)
 {
 #pragma clang fp contract(off)
@@ -130,56 +142,107 @@ __global__ void k_tmat(cplx *T, const double *bes, const double *ds, int znum, i
     int r = (int)(i % per);
     long zq = i / per;
     int l1 = r % K, l = (r / K) % K, m = r / (K * K);
-    cplx out = {0.0, 0.0};
-    if (l >= m && l1 >= m) {
-        int lo = l < l1 ? l : l1, hi = l < l1 ? l1 : l;
-        const double *b = bes + zq * N;
-        const double *d = ds + ((long)(lo * (lo + 1) + m) * K + hi) * N;
-        double sg = (m % 2) ? -1.0 : 1.0;
-        const double ipr[4] = {1.0, 0.0, -1.0, 0.0}, ipi[4] = {0.0, 1.0, 0.0, -1.0};
-        double re = 0.0, im = 0.0;
-        for (int p = hi - lo; p <= lo + hi; p++) {
-            double v = sg * d[p] * b[p];
-            re += ipr[p % 4] * v;
-            im += ipi[p % 4] * v;
-        }
-        out.re = re;
-        out.im = im;
+    if (l < m || l1 < m) return;
+    int lo = l < l1 ? l : l1, hi = l < l1 ? l1 : l;
+    const double *b = bes + zq * N;
+    const double *d = ds + ((long)(lo * (lo + 1) + m) * K + hi) * N;
+    double sg = (m % 2) ? -1.0 : 1.0;
+    const double ipr[4] = {1.0, 0.0, -1.0, 0.0}, ipi[4] = {0.0, 1.0, 0.0, -1.0};
+    double re = 0.0, im = 0.0;
+    for (int p = hi - lo; p <= lo + hi; p++) {
+        double v = sg * d[p] * b[p];
+        re += ipr[p % 4] * v;
+        im += ipi[p % 4] * v;
     }
-    T[i] = out;
+    cplx out;
+    out.re = re;
+    out.im = im;
+    T[zq * tpk_sq(K) + tpk_row(m, l, K) + (l1 - m)] = out;
 }
 
-/* rotated coefficients: out[((b*ng + g)*Q + q)*3 + X][lm] */
-__global__ void k_rotate(cplx *out, const double *src_re, const double *src_im,
-                         const double *dW, const double *twg, int nbeta, int ng, int Q,
-                         int L, int conj_src)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FSX_WAVE);
+    return v;
+}
+
+/* x mod n in [0, n) for |x| < 2^24, n >= 1: a float quotient estimate
+ * (off by at most one) and one correction. */
+__device__ __forceinline__ int mod_small(int x, int n, float inv_n)
+{
+    int k = x - n * (int)floorf((float)x * inv_n);
+    if (k < 0) k += n;
+    else if (k >= n) k -= n;
+    return k;
+}
+
+/* ---- rotated receptor / ligand coefficients -------------------------------
+ * k_twiddle: src~[q][X][lm][g] = e^{-i m1 g} src^X_{l m1}(q) (conjugated first
+ * for the ligand), gamma innermost.
+ * k_rotate: out^X_{lm}(beta, gamma, q) = sum_{m1} d^l_{m m1}(beta) src~[q][X][l m1][gamma]
+ * with one wavefront per (beta, q, X, entry) and the lanes over gamma, so the
+ * Wigner-d value of a term is the same for the whole wavefront and the
+ * source loads are contiguous.  Output layout per m-chunk c (entries e in
+ * m-major order): out[coff[c] + ((q*nrows + beta*ng + gamma)*3 + X)*csize[c] + e - cstart[c]]. */
+__global__ void k_twiddle(cplx *dst, const double *src_re, const double *src_im,
+                          const double *twg, int ng, int Q, int L, int conj_src)
+{
+    const int K2 = (L + 1) * (L + 1);
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)Q * 3 * K2 * ng) return;
+    int g = (int)(i % ng);
+    long r = i / ng;
+    int lm = (int)(r % K2);
+    long xq = r / K2;   /* q*3 + X */
+    int X = (int)(xq % 3), q = (int)(xq / 3);
+    int l = lm_l(lm), m1 = lm - l * (l + 1);
+    int k = mod_small(m1 * g, ng, 1.0f / (float)ng);
+    double wr = twg[2 * k], wi = twg[2 * k + 1];
+    long so = ((long)X * Q + q) * K2 + lm;
+    double sr = src_re[so], si = (conj_src ? -1.0 : 1.0) * src_im[so];
+    cplx v;
+    v.re = wr * sr - wi * si;
+    v.im = wr * si + wi * sr;
+    dst[i] = v;
+}
+
+__global__ void __launch_bounds__(256)
+k_rotate(cplx *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
+         int nbeta, int ng, int Q, int L)
 {
     const int K2 = (L + 1) * (L + 1), N = 2 * L + 1;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long tot = (long)nbeta * ng * Q * 3 * K2;
-    if (i >= tot) return;
-    int lm = (int)(i % K2);
-    long r = i / K2;
-    int X = (int)(r % 3); r /= 3;
-    int q = (int)(r % Q); r /= Q;
-    int g = (int)(r % ng);
-    int b = (int)(r / ng);
-    int l = lm_l(lm), m = lm - l * (l + 1);
-    const double *d = dW + (long)b * (L + 1) * N * N + ((long)l * N + (L + m)) * N + L;
-    const double *sr = src_re + ((long)X * Q + q) * K2 + l * (l + 1);
-    const double *si = src_im + ((long)X * Q + q) * K2 + l * (l + 1);
-    double sgn = conj_src ? -1.0 : 1.0;
-    double re = 0.0, im = 0.0;
-    for (int m1 = -l; m1 <= l; m1++) {
-        int k = pos_mod((long)m1 * g, ng);
-        double wr = twg[2 * k], wi = twg[2 * k + 1];
-        double dv = d[m1];
-        double ar = dv * sr[m1], ai = sgn * dv * si[m1];
-        re += wr * ar - wi * ai;
-        im += wr * ai + wi * ar;
+    const int *ent_l = ent, *ent_m = ent + K2, *ent_c = ent + 2 * K2, *ent_ec = ent + 3 * K2;
+    const int *csize = ent + 4 * K2;
+    const int nrows = nbeta * ng;
+    const int lane = threadIdx.x % FSX_WAVE, wave = threadIdx.x / FSX_WAVE;
+    const int nwave = blockDim.x / FSX_WAVE;
+    /* block = (beta, q, X) */
+    const int X = blockIdx.x % 3;
+    const int q = (blockIdx.x / 3) % Q;
+    const int b = blockIdx.x / (3 * Q);
+    const double *d = dW + (long)b * (L + 1) * N * N;
+    const cplx *sq = srct + ((long)q * 3 + X) * K2 * ng;
+    for (int e = wave; e < K2; e += nwave) {
+        const int l = ent_l[e], m = ent_m[e];
+        const double *drow = d + ((long)l * N + (L + m)) * N + L;
+        const cplx *a = sq + (long)(l * (l + 1)) * ng;
+        const int c = ent_c[e];
+        cplx *o = out + coff[c] + ent_ec[e];
+        for (int g = lane; g < ng; g += FSX_WAVE) {
+            double re = 0.0, im = 0.0;
+            for (int m1 = -l; m1 <= l; m1++) {
+                double dv = drow[m1];
+                cplx t = a[(long)m1 * ng + g];
+                re += dv * t.re;
+                im += dv * t.im;
+            }
+            cplx v;
+            v.re = re;
+            v.im = im;
+            o[(((long)q * nrows + (long)b * ng + g) * 3 + X) * csize[c]] = v;
+        }
     }
-    out[i].re = re;
-    out[i].im = im;
 }
 
 /* ---- pose bucketing --------------------------------------------------- */
@@ -199,39 +262,51 @@ __global__ void k_fill_int(int *p, long n, int v)
     if (i < n) p[i] = v;
 }
 
+/* bucket key (z, beta2, gamma2) and the pose's receptor row / alpha index */
 __global__ void k_count(const long long *poses, int n, int znum, int nbeta, int na, int ng,
-                        int *keys, int *counts)
+                        int *keys, int2 *info, int *counts)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     long long idx = poses[i];
     int key = -1;
+    int2 inf = make_int2(0, 0);
     if (idx >= 0) {
         int o[6];
         decode_pose(idx, nbeta, na, ng, o);
         if (o[0] < znum) {
             key = (o[0] * nbeta + o[2]) * ng + o[5];
+            inf = make_int2(o[1] * ng + o[4], o[3]);
             atomicAdd(&counts[key], 1);
         }
     }
     keys[i] = key;
+    info[i] = inf;
 }
 
-/* exclusive scan of counts[0..nk) into offs[0..nk]; one block of 1024 */
+/* exclusive scan of counts[0..nk) into offs[0..nk]; offs[nk+1] = number of
+ * non-empty buckets.  One block of 1024. */
 __global__ void k_scan(const int *counts, int nk, int *offs)
 {
     __shared__ int part[1024];
+    __shared__ int nz[1024];
     int t = threadIdx.x;
     int chunk = (nk + 1023) / 1024;
     int b = t * chunk, e = b + chunk < nk ? b + chunk : nk;
-    int s = 0;
-    for (int i = b; i < e; i++) s += counts[i];
+    int s = 0, z = 0;
+    for (int i = b; i < e; i++) {
+        s += counts[i];
+        z += counts[i] != 0;
+    }
     part[t] = s;
+    nz[t] = z;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
         int v = t >= off ? part[t - off] : 0;
+        int w = t >= off ? nz[t - off] : 0;
         __syncthreads();
         part[t] += v;
+        nz[t] += w;
         __syncthreads();
     }
     int run = part[t] - s;
@@ -239,153 +314,269 @@ __global__ void k_scan(const int *counts, int nk, int *offs)
         offs[i] = run;
         run += counts[i];
     }
-    if (t == 1023) offs[nk] = part[1023];
+    if (t == 1023) {
+        offs[nk] = part[1023];
+        offs[nk + 1] = nz[1023];
+    }
 }
 
-__global__ void k_scatter(const int *keys, int n, const int *offs, int *fill, int *sorted)
+/* sorted pose records: x = input position, y = receptor row, z = alpha index */
+__global__ void k_scatter(const int *keys, const int2 *info, int n, const int *offs, int *fill,
+                          int4 *recs)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int k = keys[i];
     if (k < 0) return;
     int p = offs[k] + atomicAdd(&fill[k], 1);
-    sorted[p] = i;
+    int2 inf = info[i];
+    recs[p] = make_int4(i, inf.x, inf.y, 0);
 }
 
-/* ---- per-pose cross terms ------------------------------------------------ */
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FSX_WAVE);
-    return v;
-}
-
-/* cross2[(q*6 + k)*n + s] = 2 Re F_k(q) of the pose at sorted position s */
-__global__ void __launch_bounds__(FSX_CROSS_THREADS)
-k_cross(double *cross2, const cplx *Rt, const cplx *Bt, const cplx *T, const double *twa,
-        const int *counts, const int *offs, const int *sorted, const long long *poses, int n,
-        int nkeys, int Q, int L, int nbeta, int na, int ng, long total)
+/* ---- per-pose cross terms, one m-chunk per launch --------------------------
+ * Work item = (q, bucket), buckets ordered (beta2, gamma2, z) so that the
+ * items in flight at one time share q (the receptor chunk table R[c][q][*]
+ * stays in each XCD's L2) and a handful of ligand rows.  One wavefront per
+ * block walks items grid-stride.  Per item:
+ *   U^Y_e = sum_{l1 >= |m|} conj(T^{|m|}_{l l1}(q z)) B^Y_{(l1, m)}   (LDS)
+ * then per pose 2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e]) over the
+ * chunk's entries e = (l, m), added into acc[(s*Q + q)*6 + k]. */
+#define FSX_CROSS_BLOCK 256
+/* Work tile = (q, beta2, gamma2) with all z inside: the ligand row chunk
+ * (3 * nc entries) is staged in LDS once per tile, the packed translation
+ * block of each (z, q) once per z (TST; from global memory when it does not
+ * fit), and each wavefront prefetches the receptor row of its first pose
+ * before U is formed.  Then per non-empty bucket (z, beta2, gamma2):
+ *   U^Y_e = sum_{l1 >= |m|} conj(T^{|m|}_{l l1}(q z)) B^Y_{(l1, m)}   (one entry per thread)
+ * and per pose (one wavefront, J entries per lane)
+ *   2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e])
+ * over the chunk's entries e = (l, m), stored (first chunk) or added into
+ * acc[(s*Q + q)*6 + k]. */
+template <int J, int TST>
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK)
+k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double *twa,
+        const int *ent, int cstart, int nc, const int *counts, const int *offs,
+        const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int na, int ng,
+        int zn, int first)
 {
     extern __shared__ cplx smem[];
-    const int K = L + 1, K2 = K * K;
-    /* XCD-aware remap: the dispatcher deals consecutive blocks round-robin to
-     * the 8 XCDs; give each XCD a contiguous range of (q, bucket) items so the
-     * T(z, q) and B(beta2, gamma2, q) blocks they share stay in its L2. */
-    long nb = gridDim.x;
-    long per = nb / FSX_NXCD;
-    long w = (long)(blockIdx.x % FSX_NXCD) * per + blockIdx.x / FSX_NXCD;
-    if (w >= total) return;
-    int key = (int)(w % nkeys);
-    int q = (int)(w / nkeys);
-    int cnt = counts[key];
-    if (cnt == 0) return;
-    int start = offs[key];
-    int g2 = key % ng;
-    int b2 = (key / ng) % nbeta;
-    int z = key / (ng * nbeta);
-    cplx *sB = smem;            /* [3][K2] */
-    cplx *sU = smem + 3 * K2;   /* [3][K2] */
-    const cplx *bsrc = Bt + (((long)b2 * ng + g2) * Q + q) * 3 * K2;
-    for (int i = threadIdx.x; i < 3 * K2; i += blockDim.x) sB[i] = bsrc[i];
-    __syncthreads();
-    const cplx *Tq = T + ((long)z * Q + q) * K * K2;
-    for (int lm = threadIdx.x; lm < K2; lm += blockDim.x) {
-        int l = lm_l(lm), m = lm - l * (l + 1), am = m < 0 ? -m : m;
-        const cplx *Tr = Tq + ((long)am * K + l) * K;
-        double ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
-        for (int l1 = am; l1 <= L; l1++) {
-            cplx t = Tr[l1];
-            int j = l1 * (l1 + 1) + m;
-#pragma unroll
-            for (int Y = 0; Y < 3; Y++) {
-                cplx b = sB[Y * K2 + j];
-                /* conj(t) * b */
-                ur[Y] += t.re * b.re + t.im * b.im;
-                ui[Y] += t.re * b.im - t.im * b.re;
-            }
-        }
-#pragma unroll
-        for (int Y = 0; Y < 3; Y++) {
-            sU[Y * K2 + lm].re = ur[Y];
-            sU[Y * K2 + lm].im = ui[Y];
-        }
+    cplx *sB = smem;            /* [3][nc] */
+    cplx *sU = smem + 3 * nc;   /* [3][nc] */
+    cplx *sT = smem + 6 * nc;   /* packed T block (TST) */
+    const int K = L + 1, K2 = K * K, Tn = tpk_sq(K);
+    const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
+    const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
+    const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
+    const float inv_na = 1.0f / (float)na;
+    /* per-thread U entry, per-lane pairing entries */
+    const bool uact = tid < nc;
+    int ul = 0, um = 0, uam = 0;
+    if (uact) {
+        ul = ent_l[cstart + tid];
+        um = ent_m[cstart + tid];
+        uam = um < 0 ? -um : um;
     }
-    __syncthreads();
-    const int wave = threadIdx.x / FSX_WAVE, lane = threadIdx.x % FSX_WAVE;
-    const int nwaves = blockDim.x / FSX_WAVE;
-    for (int pi = wave; pi < cnt; pi += nwaves) {
-        int s = start + pi;
-        int pose = sorted[s];
-        int o[6];
-        decode_pose(poses[pose], nbeta, na, ng, o);
-        int b1 = o[1], a2 = o[3], g1 = o[4];
-        const cplx *R = Rt + (((long)b1 * ng + g1) * Q + q) * 3 * K2;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        for (int lm = lane; lm < K2; lm += FSX_WAVE) {
-            int l = lm_l(lm), m = lm - l * (l + 1);
-            int kk = pos_mod((long)m * a2, na);
-            double wr = twa[2 * kk], wi = twa[2 * kk + 1];
-            cplx rv = R[lm], rd = R[K2 + lm], rw = R[2 * K2 + lm];
-            cplx uv = sU[lm], ud = sU[K2 + lm], uw = sU[2 * K2 + lm];
-            /* twiddled receptor factor: w * R^X */
-            double vr = wr * rv.re - wi * rv.im, vi = wr * rv.im + wi * rv.re;
-            double dr = wr * rd.re - wi * rd.im, di = wr * rd.im + wi * rd.re;
-            double xr = wr * rw.re - wi * rw.im, xi = wr * rw.im + wi * rw.re;
-            /* Re of products */
-            acc[0] += vr * uv.re - vi * uv.im;
-            acc[1] += (vr * ud.re - vi * ud.im) + (dr * uv.re - di * uv.im);
-            acc[2] += (vr * uw.re - vi * uw.im) + (xr * uv.re - xi * uv.im);
-            acc[3] += dr * ud.re - di * ud.im;
-            acc[4] += (dr * uw.re - di * uw.im) + (xr * ud.re - xi * ud.im);
-            acc[5] += xr * uw.re - xi * uw.im;
-        }
+    const int ub0 = uact ? mstart[um + L] - cstart : 0;
+    const int urow = uact ? tpk_row(uam, ul, K) : 0;
+    int em[J];
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[k] = wave_sum(acc[k]);
-        if (lane < 6) {
-            double v = acc[0];
+    for (int j = 0; j < J; j++) {
+        int ec = lane + FSX_WAVE * j;
+        em[j] = ec < nc ? ent_m[cstart + ec] : 0;
+    }
+    const int slot = lane >> 3;
+    const bool writer = (lane & 7) == 0 && slot < 6;
+    const int nbg = nbeta * ng;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int q = (int)(tile / nbg), bg = (int)(tile - (long)q * nbg);
+        const int b2 = bg / ng, g2 = bg - (bg / ng) * ng;
+        const cplx *Bq = Bc + (((long)q * nrows + bg) * 3) * nc;
+        bool staged = false;
+        for (int z = 0; z < zn; z++) {
+            const int key = (z * nbeta + b2) * ng + g2;
+            const int cnt = counts[key];
+            if (cnt == 0) continue;
+            const int start = offs[key];
+            const cplx *Tq = T + ((long)z * Q + q) * Tn;
+            /* pose loader: receptor row entries, twiddles, accumulator */
+            cplx rr[3][J];
+            double wr[J], wi[J], old = 0.0;
+            int s = 0;
+            auto load_pose = [&](int p) {
+                s = start + p;
+                int4 rec = recs[s];
+                const cplx *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
 #pragma unroll
-            for (int k = 1; k < 6; k++) v = (lane == k) ? acc[k] : v;
-            cross2[((long)q * 6 + lane) * n + s] = 2.0 * v;
+                for (int j = 0; j < J; j++) {
+                    int ec = lane + FSX_WAVE * j;
+                    if (ec < nc) {
+                        int kk = mod_small(em[j] * rec.z, na, inv_na);
+                        wr[j] = twa[2 * kk];
+                        wi[j] = twa[2 * kk + 1];
+#pragma unroll
+                        for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + ec];
+                    }
+                }
+                old = (!first && writer) ? acc[((long)s * Q + q) * 6 + slot] : 0.0;
+            };
+            if (wave < cnt) load_pose(wave);
+            if (!staged) {
+                for (int i = tid; i < 3 * nc; i += FSX_CROSS_BLOCK) sB[i] = Bq[i];
+                staged = true;
+            }
+            if (TST)
+                for (int i = tid; i < Tn; i += FSX_CROSS_BLOCK) sT[i] = Tq[i];
+            __syncthreads();
+            if (uact) {
+                const cplx *Tr = (TST ? sT : Tq) + urow;
+                const cplx *bp = sB + ub0;
+                const int nk = L - uam + 1;
+                double ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+                for (int k = 0; k < nk; k++) {
+                    cplx tv = Tr[k];
+#pragma unroll
+                    for (int Y = 0; Y < 3; Y++) {
+                        cplx bv = bp[Y * nc + k];
+                        /* conj(t) * b */
+                        ur[Y] += tv.re * bv.re + tv.im * bv.im;
+                        ui[Y] += tv.re * bv.im - tv.im * bv.re;
+                    }
+                }
+#pragma unroll
+                for (int Y = 0; Y < 3; Y++) {
+                    sU[Y * nc + tid].re = ur[Y];
+                    sU[Y * nc + tid].im = ui[Y];
+                }
+            }
+            __syncthreads();
+            for (int p = wave; p < cnt; p += nwave) {
+                if (p != wave) load_pose(p);
+                double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    int ec = lane + FSX_WAVE * j;
+                    if (ec < nc) {
+                        cplx rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
+                        cplx uv = sU[ec], ud = sU[nc + ec], uw = sU[2 * nc + ec];
+                        double w_r = wr[j], w_i = wi[j];
+                        /* twiddled receptor factor: w * R^X */
+                        double vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
+                        double dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
+                        double xr = w_r * rw.re - w_i * rw.im, xi = w_r * rw.im + w_i * rw.re;
+                        v[0] += vr * uv.re - vi * uv.im;
+                        v[1] += (vr * ud.re - vi * ud.im) + (dr * uv.re - di * uv.im);
+                        v[2] += (vr * uw.re - vi * uw.im) + (xr * uv.re - xi * uv.im);
+                        v[3] += dr * ud.re - di * ud.im;
+                        v[4] += (dr * uw.re - di * uw.im) + (xr * ud.re - xi * ud.im);
+                        v[5] += xr * uw.re - xi * uw.im;
+                    }
+                }
+                /* transpose-reduce over the 64 lanes: after the three halving
+                 * steps lane bits 5,4,3 select the slot, then a plain sum over
+                 * bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
+                {
+                    bool hi = (lane & 32) != 0;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        double send = hi ? v[i] : v[i + 4];
+                        double keep = hi ? v[i + 4] : v[i];
+                        v[i] = keep + __shfl_xor(send, 32, FSX_WAVE);
+                    }
+                    hi = (lane & 16) != 0;
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        double send = hi ? v[i] : v[i + 2];
+                        double keep = hi ? v[i + 2] : v[i];
+                        v[i] = keep + __shfl_xor(send, 16, FSX_WAVE);
+                    }
+                    hi = (lane & 8) != 0;
+                    {
+                        double send = hi ? v[0] : v[1];
+                        double keep = hi ? v[1] : v[0];
+                        v[0] = keep + __shfl_xor(send, 8, FSX_WAVE);
+                    }
+                    v[0] += __shfl_xor(v[0], 4, FSX_WAVE);
+                    v[0] += __shfl_xor(v[0], 2, FSX_WAVE);
+                    v[0] += __shfl_xor(v[0], 1, FSX_WAVE);
+                }
+                if (writer) {
+                    double x = 2.0 * v[0];
+                    acc[((long)s * Q + q) * 6 + slot] = first ? x : old + x;
+                }
+            }
+            __syncthreads();
         }
     }
 }
 
-/* ---- per-pose fit -------------------------------------------------------- */
-__global__ void __launch_bounds__(256)
-k_fit(const double *cross2, const double *cint, const int *sorted, const int *offs, int nkeys,
-      int n, const double *qvals, int Q, const double *a, double mult, double peak,
-      double c1_lo, double c1_hi, double c2_lo, double c2_hi, double *score, double *c1o,
-      double *c2o, double *cross_out, int *nev_out)
+/* ---- per-pose fit, lockstep over a pose queue ------------------------------
+ * Each lane owns one fit at a time.  All busy lanes of a wavefront evaluate
+ * chi^2 and its gradient together (the same instruction stream over q), then
+ * each advances its own L-BFGS-B state to the next point it needs; a lane
+ * whose fit ended stores the result and takes the next pose from the queue.
+ * Profiles: pose-major acc[(s*Q + q)*6 + k] plus the constant parts cint[k*Q
+ * + q], rescaled by peak / I(q0) (min_saxs.c:170-179). */
+__device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitpar *fp,
+                                          const double *acc, int s, int Q, double peak,
+                                          double *cross_out, int pose)
 {
 #pragma clang fp contract(off)
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int nvalid = offs[nkeys];
-    if (s >= nvalid) return;
-    int pose = sorted[s];
-    fsx_ct t;
-    t.c = cint; t.cks = Q; t.cqs = 1;
-    t.v = cross2 + s; t.vks = n; t.vqs = 6L * n;
-    t.scale = 1.0;
-    /* min_saxs.c:170-179 */
-    double sc = fsx_ct_get(&t, 0, 0) + fsx_ct_get(&t, 3, 0) + fsx_ct_get(&t, 5, 0)
-              + fsx_ct_get(&t, 2, 0) - fsx_ct_get(&t, 1, 0) - fsx_ct_get(&t, 4, 0);
+    t->v = acc + (long)s * Q * 6;
+    t->scale = 1.0;
+    double sc = fsx_ct_get(t, 0, 0) + fsx_ct_get(t, 3, 0) + fsx_ct_get(t, 5, 0)
+              + fsx_ct_get(t, 2, 0) - fsx_ct_get(t, 1, 0) - fsx_ct_get(t, 4, 0);
     if (cross_out) {
         for (int k = 0; k < 6; k++)
             for (int q = 0; q < Q; q++)
-                cross_out[((long)pose * 6 + k) * Q + q] = fsx_ct_get(&t, k, q);
+                cross_out[((long)pose * 6 + k) * Q + q] = fsx_ct_get(t, k, q);
     }
-    t.scale = peak / sc;
+    t->scale = peak / sc;
+    fsx_lb_start(S, fp, 1.0, 0.0);
+}
+
+__global__ void __launch_bounds__(256)
+k_fit(const double *acc, const double *cint, const int4 *recs, const int *nvalid_p, int Q,
+      const double *qvals, const double *a, double mult, double peak, double c1_lo,
+      double c1_hi, double c2_lo, double c2_hi, double *score, double *c1o, double *c2o,
+      double *cross_out, int *next, unsigned long long *tally)
+{
+#pragma clang fp contract(off)
+    const int nvalid = *nvalid_p;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
     fsx_fitpar fp;
     fp.a = a; fp.q = qvals; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
-    double x[2] = {1.0, 0.0}, f;
-    int nev;
-    fsx_lbfgsb_fit(&t, &fp, x, &f, &nev);
-    score[pose] = sqrt(f);
-    c1o[pose] = x[0];
-    c2o[pose] = x[1];
-    if (nev_out) nev_out[s] = nev;
+    fsx_ct t;
+    t.c = cint; t.cks = Q; t.cqs = 1;
+    t.v = acc; t.vks = 1; t.vqs = 6;
+    t.scale = 1.0;
+    fsx_lbrc S;
+    unsigned long long nev = 0, ncap = 0;
+    int act = s < nvalid;
+    int pose = act ? recs[s].x : 0;
+    if (act) fit_begin(&t, &S, &fp, acc, s, Q, peak, cross_out, pose);
+    while (__any(act)) {
+        if (act) {
+            double k;
+            lb_eval(&t, &fp, S.x, &S.f, S.g, &k);
+        }
+        if (act && !fsx_lb_resume(&S)) {
+            score[pose] = sqrt(S.f);
+            c1o[pose] = S.x[0];
+            c2o[pose] = S.x[1];
+            nev += S.nev;
+            ncap += S.task == FSX_T_STOP_ITER;
+            s = atomicAdd(next, 1);
+            act = s < nvalid;
+            if (act) {
+                pose = recs[s].x;
+                fit_begin(&t, &S, &fp, acc, s, Q, peak, cross_out, pose);
+            }
+        }
+    }
+    if (nev) atomicAdd(&tally[0], nev);
+    if (ncap) atomicAdd(&tally[1], ncap);
 }
 
 /* fits of caller-supplied (rescaled) profiles ct[n][6][Q] */
@@ -477,9 +668,9 @@ static int g_init = 0;
 static hipStream_t g_stream;
 enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
-    B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_SORTED, B_CROSS2, B_SCORE, B_C1, B_C2,
-    B_CROSSOUT, B_NEV, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF, B_AY, B_AJ, B_AQ,
-    B_ARE, B_AIM, B_NBUF
+    B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
+    B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
+    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -510,11 +701,26 @@ static int ensure(int slot, size_t bytes)
     return FSX_OK;
 }
 
+static int g_ncu = 256;
+
+/* launch-shape knobs: environment override for tuning, defaults otherwise */
+static int env_int(const char *name, int dflt, int lo, int hi)
+{
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    int x = atoi(v);
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
 static int init_once(void)
 {
     if (g_init) return FSX_OK;
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return FSX_ERR_NODEV;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (g_ncu < 1) g_ncu = 1;
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_init = 1;
     return FSX_OK;
@@ -546,59 +752,146 @@ static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t);
         if (r_ != FSX_OK) return r_;   \
     } while (0)
 
+/* Entry tables of the m-major coefficient order used by the rotated tables
+ * and the cross-term kernel: e runs over m = -L..L, then l = |m|..L.  The
+ * entries are cut into C chunks of whole m values with about K2/C entries
+ * each.  ent layout (ints): [0,K2) l, [K2,2K2) m, [2K2,3K2) chunk,
+ * [3K2,4K2) index within the chunk, [4K2, 4K2+C) chunk sizes,
+ * [5K2, 5K2+2L+1) first entry of each m, [6K2, 6K2+C) chunk starts. */
+static void entry_tables(int L, int C, int *ent)
+{
+    const int K2 = (L + 1) * (L + 1);
+    int *el = ent, *em = ent + K2, *ec = ent + 2 * K2, *ei = ent + 3 * K2;
+    int *csz = ent + 4 * K2, *mst = ent + 5 * K2, *cst = ent + 6 * K2;
+    int e = 0, c = 0, cfirst = 0;
+    for (int m = -L; m <= L; m++) {
+        int am = m < 0 ? -m : m;
+        /* close the chunk once it holds its share of the entries */
+        if (e > cfirst && c < C - 1 && (long)e * C >= (long)(c + 1) * K2) {
+            csz[c] = e - cfirst;
+            cst[c] = cfirst;
+            c++;
+            cfirst = e;
+        }
+        mst[m + L] = e;
+        for (int l = am; l <= L; l++, e++) {
+            el[e] = l;
+            em[e] = m;
+            ec[e] = c;
+            ei[e] = e - cfirst;
+        }
+    }
+    csz[c] = e - cfirst;
+    cst[c] = cfirst;
+    for (int k = c + 1; k < C; k++) { csz[k] = 0; cst[k] = e; }
+}
+
 static int run_locked(const fsx_hip_args *A, fsx_stats *st)
 {
     RC(init_once());
     const int L = A->L, K = L + 1, K2 = K * K, N = 2 * L + 1, Q = A->qnum, n = A->n;
     const int nb = A->nbeta, na = A->nalpha, ng = A->ngamma, zn = A->znum;
+    const int nrows = nb * ng;
     const long nkeys = (long)zn * nb * ng;
-    if (nkeys > 0x7fffffffL) return FSX_ERR_ARG;
+    if (nkeys > 0x3fffffffL || (long)nrows * Q > 0x7fffffffL) return FSX_ERR_ARG;
+    if ((long)L * na >= (1L << 24) || (long)L * ng >= (1L << 24)) return FSX_ERR_ARG;
     const size_t dsz = (size_t)K * N * N;
     const size_t spf = (size_t)3 * Q * K2;
-    size_t lds = sizeof(cplx) * 6 * K2;
-    if (lds > 160 * 1024) return FSX_ERR_ARG;
     hipStream_t s = g_stream;
-    RC(ensure(B_BES, sizeof(double) * zn * Q * N));
-    RC(ensure(B_T, sizeof(cplx) * zn * Q * (size_t)K * K2));
-    RC(ensure(B_RT, sizeof(cplx) * (size_t)nb * ng * spf));
-    RC(ensure(B_BT, sizeof(cplx) * (size_t)nb * ng * spf));
-    RC(ensure(B_A, sizeof(double) * 2 * spf));
-    RC(ensure(B_B, sizeof(double) * 2 * spf));
-    RC(ensure(B_DW, sizeof(double) * dsz * nb));
-    RC(ensure(B_DS, sizeof(double) * (size_t)K * K2 * N));
-    RC(ensure(B_TWA, sizeof(double) * 2 * na));
-    RC(ensure(B_TWG, sizeof(double) * 2 * ng));
-    RC(ensure(B_Q, sizeof(double) * Q));
-    RC(ensure(B_Z, sizeof(double) * zn));
-    RC(ensure(B_AM, sizeof(double) * 6 * Q));
-    RC(ensure(B_CINT, sizeof(double) * 6 * Q));
-    RC(ensure(B_KEYS, sizeof(int) * n));
-    RC(ensure(B_COUNTS, sizeof(int) * nkeys));
-    RC(ensure(B_OFFS, sizeof(int) * (nkeys + 1)));
-    RC(ensure(B_FILL, sizeof(int) * nkeys));
-    RC(ensure(B_SORTED, sizeof(int) * n));
-    RC(ensure(B_CROSS2, sizeof(double) * 6 * Q * (size_t)n));
-    RC(ensure(B_NEV, sizeof(int) * n));
+    /* m-chunks: one launch per chunk of entries; one chunk unless a chunk
+     * would exceed 4 entries per lane (FSX_CROSS_CHUNKS overrides) */
+    int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
+    int *ent = (int *)malloc(sizeof(int) * (7 * K2 + 2));
+    long *coff = (long *)malloc(sizeof(long) * (N + 1));
+    if (!ent || !coff) {
+        free(ent);
+        free(coff);
+        return FSX_ERR_NOMEM;
+    }
+    /* at most 4 entries per lane: more chunks until the largest fits */
+    int ncmax;
+    for (;;) {
+        entry_tables(L, C, ent);
+        ncmax = 0;
+        for (int c = 0; c < C; c++)
+            if (ent[4 * K2 + c] > ncmax) ncmax = ent[4 * K2 + c];
+        if (ncmax <= 4 * FSX_WAVE || C == N) break;
+        C++;
+    }
+    if (ncmax > 4 * FSX_WAVE) {
+        free(ent);
+        free(coff);
+        return FSX_ERR_ARG;
+    }
+    const int *csz = ent + 4 * K2, *cst = ent + 6 * K2;
+    coff[0] = 0;
+    for (int c = 0; c < C; c++) coff[c + 1] = coff[c] + (long)Q * nrows * 3 * csz[c];
+    int rc = FSX_OK;
+#define RCF(x)                             \
+    do {                                   \
+        rc = (x);                          \
+        if (rc != FSX_OK) goto out;        \
+    } while (0)
+#define HIPF(x)                                                                       \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            fprintf(stderr, "[fsx] %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                              \
+            rc = FSX_ERR_HIP;                                                         \
+            goto out;                                                                 \
+        }                                                                             \
+    } while (0)
+    {
+    RCF(ensure(B_BES, sizeof(double) * zn * Q * N));
+    const int Tn = tpk_sq(K);
+    RCF(ensure(B_T, sizeof(cplx) * zn * Q * (size_t)Tn));
+    RCF(ensure(B_RT, sizeof(cplx) * (size_t)nrows * spf));
+    RCF(ensure(B_BT, sizeof(cplx) * (size_t)nrows * spf));
+    RCF(ensure(B_A, sizeof(double) * 2 * spf));
+    RCF(ensure(B_B, sizeof(double) * 2 * spf));
+    RCF(ensure(B_DW, sizeof(double) * dsz * nb));
+    RCF(ensure(B_DS, sizeof(double) * (size_t)K * K2 * N));
+    RCF(ensure(B_TWA, sizeof(double) * 2 * na));
+    RCF(ensure(B_TWG, sizeof(double) * 2 * ng));
+    RCF(ensure(B_Q, sizeof(double) * Q));
+    RCF(ensure(B_Z, sizeof(double) * zn));
+    RCF(ensure(B_AM, sizeof(double) * 6 * Q));
+    RCF(ensure(B_CINT, sizeof(double) * 6 * Q));
+    RCF(ensure(B_KEYS, sizeof(int) * n));
+    RCF(ensure(B_INFO, sizeof(int2) * n));
+    RCF(ensure(B_COUNTS, sizeof(int) * nkeys));
+    RCF(ensure(B_OFFS, sizeof(int) * (nkeys + 2)));
+    RCF(ensure(B_FILL, sizeof(int) * nkeys));
+    RCF(ensure(B_RECS, sizeof(int4) * n));
+    RCF(ensure(B_ACC, sizeof(double) * 6 * Q * (size_t)n));
+    RCF(ensure(B_ENT, sizeof(int) * (7 * K2 + 2)));
+    RCF(ensure(B_COFF, sizeof(long) * (C + 1)));
+    RCF(ensure(B_NEXT, sizeof(int)));
+    RCF(ensure(B_TALLY, 2 * sizeof(unsigned long long)));
+    RCF(ensure(B_SRCT, sizeof(cplx) * (size_t)Q * 3 * K2 * ng));
     double *dA = (double *)g_buf[B_A], *dB = (double *)g_buf[B_B];
-    HIPCHK(hipMemcpyAsync(dA, A->A_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dA + spf, A->A_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dB, A->B_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dB + spf, A->B_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_DW], A->dW, sizeof(double) * dsz * nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_TWA], A->tw_a, sizeof(double) * 2 * na, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_TWG], A->tw_g, sizeof(double) * 2 * ng, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_Q], A->qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_Z], A->zvals, sizeof(double) * zn, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_AM], A->a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(g_buf[B_CINT], A->cint, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(dA, A->A_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(dA + spf, A->A_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(dB, A->B_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(dB + spf, A->B_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_DW], A->dW, sizeof(double) * dsz * nb, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_TWA], A->tw_a, sizeof(double) * 2 * na, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_TWG], A->tw_g, sizeof(double) * 2 * ng, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_Q], A->qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_Z], A->zvals, sizeof(double) * zn, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_AM], A->a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_CINT], A->cint, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_ENT], ent, sizeof(int) * (7 * K2 + 2), hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_COFF], coff, sizeof(long) * (C + 1), hipMemcpyHostToDevice, s));
 
     const long long *dposes;
     if (A->flags & FSX_DEVICE_POSES) {
         dposes = (const long long *)A->poses;
     } else {
-        RC(ensure(B_POSES, sizeof(long long) * n));
-        HIPCHK(hipMemcpyAsync(g_buf[B_POSES], A->poses, sizeof(long long) * n, hipMemcpyHostToDevice, s));
+        RCF(ensure(B_POSES, sizeof(long long) * n));
+        HIPF(hipMemcpyAsync(g_buf[B_POSES], A->poses, sizeof(long long) * n, hipMemcpyHostToDevice, s));
         dposes = (const long long *)g_buf[B_POSES];
     }
     double *dscore, *dc1, *dc2, *dcross = NULL;
@@ -608,23 +901,23 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     } else {
         /* device mirrors start from the caller's values so that poses the
          * reference leaves untouched stay untouched */
-        RC(ensure(B_SCORE, sizeof(double) * n));
-        RC(ensure(B_C1, sizeof(double) * n));
-        RC(ensure(B_C2, sizeof(double) * n));
+        RCF(ensure(B_SCORE, sizeof(double) * n));
+        RCF(ensure(B_C1, sizeof(double) * n));
+        RCF(ensure(B_C2, sizeof(double) * n));
         dscore = (double *)g_buf[B_SCORE]; dc1 = (double *)g_buf[B_C1]; dc2 = (double *)g_buf[B_C2];
-        HIPCHK(hipMemcpyAsync(dscore, A->score, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(dc1, A->c1, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(dc2, A->c2, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPF(hipMemcpyAsync(dscore, A->score, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPF(hipMemcpyAsync(dc1, A->c1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPF(hipMemcpyAsync(dc2, A->c2, sizeof(double) * n, hipMemcpyHostToDevice, s));
         if (A->cross) {
-            RC(ensure(B_CROSSOUT, sizeof(double) * 6 * Q * (size_t)n));
+            RCF(ensure(B_CROSSOUT, sizeof(double) * 6 * Q * (size_t)n));
             dcross = (double *)g_buf[B_CROSSOUT];
-            HIPCHK(hipMemcpyAsync(dcross, A->cross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyHostToDevice, s));
+            HIPF(hipMemcpyAsync(dcross, A->cross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyHostToDevice, s));
         }
     }
 
     hipEvent_t ev[7];
-    for (int i = 0; i < 7; i++) HIPCHK(hipEventCreate(&ev[i]));
-    HIPCHK(hipEventRecord(ev[0], s));
+    for (int i = 0; i < 7; i++) HIPF(hipEventCreate(&ev[i]));
+    HIPF(hipEventRecord(ev[0], s));
     /* translation tables */
     long nbes = (long)zn * Q * N;
     hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g_buf[B_BES],
@@ -632,81 +925,121 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     long nt = (long)zn * Q * K * K2;
     hipLaunchKernelGGL(k_tmat, dim3(nblk(nt, 256)), dim3(256), 0, s, (cplx *)g_buf[B_T],
                        (const double *)g_buf[B_BES], (const double *)g_buf[B_DS], zn, Q, L);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[1], s));
-    /* rotated receptor and ligand coefficients */
-    long nr = (long)nb * ng * (long)spf;
-    hipLaunchKernelGGL(k_rotate, dim3(nblk(nr, 256)), dim3(256), 0, s, (cplx *)g_buf[B_RT],
-                       (const double *)dA, (const double *)(dA + spf), (const double *)g_buf[B_DW],
-                       (const double *)g_buf[B_TWG], nb, ng, Q, L, 0);
-    hipLaunchKernelGGL(k_rotate, dim3(nblk(nr, 256)), dim3(256), 0, s, (cplx *)g_buf[B_BT],
-                       (const double *)dB, (const double *)(dB + spf), (const double *)g_buf[B_DW],
-                       (const double *)g_buf[B_TWG], nb, ng, Q, L, 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[2], s));
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[1], s));
+    /* rotated receptor and ligand coefficients, m-chunked */
+    {
+        long ntw = (long)Q * 3 * K2 * ng;
+        cplx *srct = (cplx *)g_buf[B_SRCT];
+        hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
+                           (const double *)dA, (const double *)(dA + spf),
+                           (const double *)g_buf[B_TWG], ng, Q, L, 0);
+        hipLaunchKernelGGL(k_rotate, dim3((unsigned)(nb * Q * 3)), dim3(256), 0, s,
+                           (cplx *)g_buf[B_RT], (const cplx *)srct, (const double *)g_buf[B_DW],
+                           (const int *)g_buf[B_ENT], (const long *)g_buf[B_COFF], nb, ng, Q, L);
+        hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
+                           (const double *)dB, (const double *)(dB + spf),
+                           (const double *)g_buf[B_TWG], ng, Q, L, 1);
+        hipLaunchKernelGGL(k_rotate, dim3((unsigned)(nb * Q * 3)), dim3(256), 0, s,
+                           (cplx *)g_buf[B_BT], (const cplx *)srct, (const double *)g_buf[B_DW],
+                           (const int *)g_buf[B_ENT], (const long *)g_buf[B_COFF], nb, ng, Q, L);
+    }
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[2], s));
     /* bucket poses by (z, beta2, gamma2) */
     int *counts = (int *)g_buf[B_COUNTS], *offs = (int *)g_buf[B_OFFS], *fill = (int *)g_buf[B_FILL];
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, counts, nkeys, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, fill, nkeys, 0);
     hipLaunchKernelGGL(k_count, dim3(nblk(n, 256)), dim3(256), 0, s, dposes, n, zn, nb, na, ng,
-                       (int *)g_buf[B_KEYS], counts);
+                       (int *)g_buf[B_KEYS], (int2 *)g_buf[B_INFO], counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const int *)counts, (int)nkeys, offs);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, (const int *)g_buf[B_KEYS],
-                       n, (const int *)offs, fill, (int *)g_buf[B_SORTED]);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[3], s));
-    /* per-pose cross terms */
-    long total = (long)Q * nkeys;
-    long grid = (total + FSX_NXCD - 1) / FSX_NXCD * FSX_NXCD;
-    if (grid > 0x7fffffffL) return FSX_ERR_ARG;
-    hipLaunchKernelGGL(k_cross, dim3((unsigned)grid), dim3(FSX_CROSS_THREADS), lds, s,
-                       (double *)g_buf[B_CROSS2], (const cplx *)g_buf[B_RT],
-                       (const cplx *)g_buf[B_BT], (const cplx *)g_buf[B_T],
-                       (const double *)g_buf[B_TWA], (const int *)counts, (const int *)offs,
-                       (const int *)g_buf[B_SORTED], dposes, n, (int)nkeys, Q, L, nb, na, ng, total);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[4], s));
-    /* per-pose fits */
-    hipLaunchKernelGGL(k_fit, dim3(nblk(n, 256)), dim3(256), 0, s, (const double *)g_buf[B_CROSS2],
-                       (const double *)g_buf[B_CINT], (const int *)g_buf[B_SORTED],
-                       (const int *)offs, (int)nkeys, n, (const double *)g_buf[B_Q], Q,
-                       (const double *)g_buf[B_AM], A->mult, A->peak, A->c1_lo, A->c1_hi,
-                       A->c2_lo, A->c2_hi, dscore, dc1, dc2, dcross, (int *)g_buf[B_NEV]);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[5], s));
-    if (!dev_out) {
-        HIPCHK(hipMemcpyAsync(A->score, dscore, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(A->c1, dc1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(A->c2, dc2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        if (A->cross)
-            HIPCHK(hipMemcpyAsync(A->cross, dcross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyDeviceToHost, s));
+                       (const int2 *)g_buf[B_INFO], n, (const int *)offs, fill,
+                       (int4 *)g_buf[B_RECS]);
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[3], s));
+    /* per-pose cross terms, one launch per m-chunk, accumulated in order */
+    const long ntiles = (long)Q * nb * ng;
+    const int tst = (long)Tn * (long)sizeof(cplx) <= 40 * 1024;
+    long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 3, 1, 16);
+    if (grid > ntiles) grid = ntiles;
+    for (int c = 0; c < C; c++) {
+        if (csz[c] == 0) continue;
+        int jn = (csz[c] + FSX_WAVE - 1) / FSX_WAVE;
+        void (*kern)(double *, const cplx *, const cplx *, const cplx *, const double *,
+                     const int *, int, int, const int *, const int *, const int4 *, long, int,
+                     int, int, int, int, int, int, int);
+        if (tst) kern = jn <= 1 ? k_cross<1, 1> : jn <= 2 ? k_cross<2, 1> : k_cross<4, 1>;
+        else kern = jn <= 1 ? k_cross<1, 0> : jn <= 2 ? k_cross<2, 0> : k_cross<4, 0>;
+        size_t lds = sizeof(cplx) * (6 * (size_t)csz[c] + (tst ? Tn : 0));
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(FSX_CROSS_BLOCK), lds, s,
+                           (double *)g_buf[B_ACC], (const cplx *)g_buf[B_RT] + coff[c],
+                           (const cplx *)g_buf[B_BT] + coff[c], (const cplx *)g_buf[B_T],
+                           (const double *)g_buf[B_TWA], (const int *)g_buf[B_ENT], cst[c], csz[c],
+                           (const int *)counts, (const int *)offs, (const int4 *)g_buf[B_RECS],
+                           ntiles, nrows, Q, L, nb, na, ng, zn, c == 0);
     }
-    int nvalid = 0;
-    HIPCHK(hipMemcpyAsync(&nvalid, offs + nkeys, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(ev[6], s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[4], s));
+    /* per-pose fits over a pose queue */
+    long lanes = (long)g_ncu * 4 * 64 * env_int("FSX_FIT_WAVES_PER_SIMD", 2, 1, 16);
+    if (lanes > n) lanes = ((long)n + 255) / 256 * 256;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_NEXT], 1L, (int)lanes);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_TALLY], 4L, 0);
+    hipLaunchKernelGGL(k_fit, dim3((unsigned)(lanes / 256)), dim3(256), 0, s,
+                       (const double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
+                       (const int4 *)g_buf[B_RECS], (const int *)(offs + nkeys), Q,
+                       (const double *)g_buf[B_Q], (const double *)g_buf[B_AM], A->mult, A->peak,
+                       A->c1_lo, A->c1_hi, A->c2_lo, A->c2_hi, dscore, dc1, dc2, dcross,
+                       (int *)g_buf[B_NEXT], (unsigned long long *)g_buf[B_TALLY]);
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[5], s));
+    if (!dev_out) {
+        HIPF(hipMemcpyAsync(A->score, dscore, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPF(hipMemcpyAsync(A->c1, dc1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPF(hipMemcpyAsync(A->c2, dc2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (A->cross)
+            HIPF(hipMemcpyAsync(A->cross, dcross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyDeviceToHost, s));
+    }
+    int tail[2] = {0, 0};
+    unsigned long long tally[2] = {0, 0};
+    HIPF(hipMemcpyAsync(tail, offs + nkeys, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIPF(hipMemcpyAsync(tally, g_buf[B_TALLY], sizeof(tally), hipMemcpyDeviceToHost, s));
+    HIPF(hipEventRecord(ev[6], s));
+    HIPF(hipStreamSynchronize(s));
     float ms[6];
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-    for (int i = 0; i < 7; i++) HIPCHK(hipEventDestroy(ev[i]));
+    for (int i = 0; i < 6; i++) HIPF(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    for (int i = 0; i < 7; i++) HIPF(hipEventDestroy(ev[i]));
+    const int nvalid = tail[0];
+    const long items = (long)Q * tail[1];
     st->ms_bessel_t = ms[0];
     st->ms_rotate = ms[1];
     st->ms_group = ms[2];
     st->ms_cross = ms[3];
     st->ms_fit = ms[4];
     st->n_scored = nvalid;
-    /* algorithmic traffic of k_cross: per (bucket, q) the ligand block and the
-     * translation rows, per pose and q the receptor block and 6 outputs */
-    long nonempty_q = 0;
-    {
-        /* count non-empty buckets from the sorted keys: nvalid poses, and the
-         * buckets themselves are bounded by min(nkeys, nvalid) */
-        nonempty_q = (long)Q * (nkeys < nvalid ? nkeys : nvalid);
+    st->fit_evals = (long)tally[0];
+    st->n_capped = (int)tally[1];
+    st->cross_items = items;
+    st->cross_chunks = C;
+    /* useful FP64 flops of the cross-term launches: 8 per complex
+     * multiply-add of the translated ligand (3 parts, sum_e (L+1-|m_e|)
+     * terms per item) and 54 per (pose, q, entry) of the pairing */
+    double tpair = 0.0;
+    for (int e = 0; e < K2; e++) tpair += K - (ent[K2 + e] < 0 ? -ent[K2 + e] : ent[K2 + e]);
+    st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * K2 * 54.0;
+    /* algorithmic bytes: per pose and q one receptor row (3 parts) and the
+     * six-term accumulator (read and written by every chunk after the
+     * first); per item one ligand row */
+    st->cross_bytes = (double)nvalid * Q * (3.0 * K2 * 16.0 + (2.0 * C - 1.0) * 48.0)
+                    + (double)items * 3.0 * K2 * 16.0;
     }
-    st->cross_bytes = (double)nonempty_q * (3.0 * K2 * 16.0 + (double)K * K2 * 16.0 / 2.0)
-                    + (double)nvalid * Q * (3.0 * K2 * 16.0 + 6.0 * 8.0);
-    st->cross_flops = (double)nonempty_q * 3.0 * 8.0 * (K2 * (K + 1) / 3.0)
-                    + (double)nvalid * Q * K2 * (12.0 + 6.0 * 4.0 + 9.0 * 2.0);
-    return FSX_OK;
+out:
+    free(ent);
+    free(coff);
+#undef RCF
+#undef HIPF
+    return rc;
 }
 
 extern "C" int fsx_hip_run(const fsx_hip_args *A, fsx_stats *st)

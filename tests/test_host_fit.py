@@ -1,0 +1,34 @@
+"""The product's host fit (the L-BFGS-B driver shared by the host entry
+points and the GPU fit kernels) against the oracle's L-BFGS-B, which is
+itself pinned bit-for-bit to the reference lbfgsb build: both run on the
+same libm here, so score, c1, c2 and the evaluation count must be
+identical.  Runs without a GPU (the host entry point makes no HIP call)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problem import build_problem
+
+
+@pytest.fixture(scope="module")
+def F():
+    import libfmftsaxs_amd as F
+    F.lib()
+    return F
+
+
+@pytest.mark.parametrize("bounds", [((0.96, 1.04), (-2.0, 4.0)), ((0.95, 1.05), (0.0, 4.0))])
+def test_host_fit_bit_identical_to_oracle(F, orc, bounds):
+    pb = build_problem(300, 200, 8, 0.3, 25, np.arange(15.0, 35.0), 300, seed=23)
+    P = O.Params(pb["a"], pb["rm"], pb["peak"], c1=bounds[0], c2=bounds[1])
+    r = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                              P, pb["qvals"], pb["zvals"], pb["L"], want_cross=True)
+    n_bound = 0
+    for i, ct in enumerate(r["cross"]):
+        sc, c1, c2, nev, cts = O.fit_one(ct, pb["qvals"], P)
+        got = F.fit_host(cts, pb["qvals"], pb["a"], P.s.mult, c1=bounds[0], c2=bounds[1])
+        assert (got[0], got[1], got[2], got[3]) == (sc, c1, c2, nev), i
+        assert (sc, c1, c2) == (r["score"][i], r["c1"][i], r["c2"][i])
+        n_bound += c1 in bounds[0] or c2 in bounds[1]
+    # the set exercises fits that end on a bound as well as interior ones
+    assert 0 < n_bound < len(r["cross"])
