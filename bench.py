@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--poses", type=int, default=210000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed", action="store_true",
+                    help="time the mixed-precision form (FP32 contraction, FP64 fit)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy "
                          "kernel of known size, to calibrate HBM byte counters)")
@@ -157,7 +159,7 @@ def main():
 
     def step():
         return F.score_device(pb, d_poses.data_ptr(), n, d_score.data_ptr(), d_c1.data_ptr(),
-                              d_c2.data_ptr())
+                              d_c2.data_ptr(), mixed=args.mixed)
 
     for i in range(args.warmup):
         st = step()

@@ -42,6 +42,10 @@ extern "C" {
 #define FSX_DEVICE_POSES 1u    /* poses points to device memory */
 #define FSX_DEVICE_OUTPUT 2u   /* score/c1/c2/cross point to device memory */
 #define FSX_ALL_POSES_VALID 4u /* caller guarantees every index is in range */
+#define FSX_MIXED_FP32 8u      /* mixed-precision form: rotated coefficients, translation
+                                  blocks and the cross-term contraction in FP32 (Bessel,
+                                  Wigner-d, the lane reductions, the per-pose accumulators
+                                  and the chi^2 fit stay FP64) */
 
 /* pi of the reference CMake build (-std=c11 leaves M_PI undefined, so
  * define.h:13-15 supplies this truncated value) */
@@ -117,6 +121,23 @@ int fsx_atoms2spf(int natoms, const double *coords, const double *vac, const dou
 
 /* Bessel table j_p(x_i) for p = 0..pmax on the GPU, x host [n]; out [n][pmax+1] */
 int fsx_sbessel_batch(const double *x, int n, int pmax, double *out);
+
+/* Correlate front end (tools/correlate.c:202-251): read an Euler list
+ * ("ft_id z b1 g1 a2 b2 g2" per line) and snap every line to the grid; a
+ * line yields one index per zvals entry within 0.001 A of its z, with the
+ * digits rounded separately and not reduced (a full turn carries into the
+ * next digit, as in the reference).  Grid: beta step pi/(nbeta-1), alpha
+ * and gamma steps 2pi/nalpha and 2pi/ngamma, alpha2 and gamma2 mirrored to
+ * 2pi - angle; pi = FSX_PI_REFERENCE reproduces the reference.  *index,
+ * *ft_id and *order (input line number) are allocated here; release them
+ * with fsx_free. */
+int fsx_euler_list_read(const char *path, const double *zvals, int znum, int L, int nbeta,
+                        int nalpha, int ngamma, double pi, int64_t **index, int **ft_id,
+                        int **order, int *n);
+/* Score file of tools/correlate.c:371-381: "order ft_id score c1 c2" lines */
+int fsx_scores_write(const char *path, const int *order, const int *ft_id, const double *score,
+                     const double *c1, const double *c2, int n);
+void fsx_free(void *p);
 
 int fsx_device_count(void);
 const char *fsx_error_string(int code);

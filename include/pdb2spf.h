@@ -9,6 +9,9 @@
 
 #include <stdio.h>
 
+#include "form_factor_table.h"
+#include "fsx_mol.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +46,21 @@ void sxs_spf_full_reset(struct sxs_spf_full *s);                 /* pdb2spf.c:25
 void sxs_spf_full_write(char *path, struct sxs_spf_full *s);     /* pdb2spf.c:267-295 */
 struct sxs_spf_full *sxs_spf_full_fread(FILE *f);                /* pdb2spf.c:297-341 */
 struct sxs_spf_full *sxs_spf_full_read(char *path);              /* pdb2spf.c:343-350 */
+
+/* pdb2spf.c:24-152: SPF of an atom group (coordinates already centred) into
+ * spf_coefs (allocated for at least L and qnum): rm = mean vdW radius, per
+ * atom the vacuum / dummy form factors from ff_table, the water shell
+ * weighted by zero_ff(OH2) * saxs_sa[i] (no water shell if saxs_sa is
+ * NULL).  The atom sums run on the GPU.  An atom without a form factor is
+ * fatal. */
+void atom_grp2spf_inplace(struct sxs_spf_full *spf_coefs, struct mol_atom_group *ag,
+                          struct saxs_form_factor_table *ff_table, double *qvals, int qnum,
+                          int L, double *saxs_sa);
+/* pdb2spf.c:154-175: as above into a new object; water == 1 computes the
+ * fractional SASA (probe 1.4 A, sxs_faccs) for the water shell */
+struct sxs_spf_full *atom_grp2spf(struct mol_atom_group *ag,
+                                  struct saxs_form_factor_table *ff_table, double *qvals,
+                                  int qnum, int L, int water);
 
 #ifdef __cplusplus
 }

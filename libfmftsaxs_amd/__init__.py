@@ -16,6 +16,7 @@ PI_REFERENCE = 3.14159265358
 
 FSX_DEVICE_POSES = 1
 FSX_DEVICE_OUTPUT = 2
+FSX_MIXED_FP32 = 8
 
 
 class FsxError(RuntimeError):
@@ -170,9 +171,10 @@ class ScoringProblem:
         return self.s.qnum
 
 
-def score(problem, poses, want_cross=False, out=None):
+def score(problem, poses, want_cross=False, out=None, mixed=False):
     """Score poses (int64 mixed-radix indices).  Returns dict with score, c1,
-    c2 (NaN where the z slot is out of range), optional cross [n,6,Q], stats."""
+    c2 (NaN where the z slot is out of range), optional cross [n,6,Q], stats.
+    mixed=True selects the mixed-precision form (FP32 contraction)."""
     poses = np.ascontiguousarray(poses, dtype=np.int64)
     n = poses.size
     sc = np.full(n, np.nan)
@@ -182,15 +184,17 @@ def score(problem, poses, want_cross=False, out=None):
     st = Stats()
     _chk(lib().fsx_score(C.byref(problem.s), poses.ctypes.data, n, sc.ctypes.data,
                          c1.ctypes.data, c2.ctypes.data,
-                         None if cross is None else cross.ctypes.data, 0, C.byref(st)))
+                         None if cross is None else cross.ctypes.data,
+                         FSX_MIXED_FP32 if mixed else 0, C.byref(st)))
     return {"score": sc, "c1": c1, "c2": c2, "cross": cross, "stats": st.as_dict()}
 
 
-def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0):
+def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0, mixed=False):
     """Device-resident form: all pointers are device addresses (ints)."""
     st = Stats()
+    flags = FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT | (FSX_MIXED_FP32 if mixed else 0)
     _chk(lib().fsx_score(C.byref(problem.s), d_poses, n, d_score, d_c1, d_c2, d_cross or None,
-                         FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT, C.byref(st)))
+                         flags, C.byref(st)))
     return st.as_dict()
 
 

@@ -104,6 +104,46 @@ FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
     }
 }
 
+/* Terms at q = i0 .. i0+nb-1 (nb <= FSX_QB) into blk: all raw loads first,
+ * then (c + v) * scale per value exactly as fsx_ct_get. */
+#define FSX_QB 4
+FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
+{
+    double cr[FSX_QB][6];
+    const double *c = t->c + i0 * t->cqs;
+    const long cks = t->cks, cqs = t->cqs;
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+    for (int u = 0; u < FSX_QB; u++)
+        for (int k = 0; k < 6; k++) cr[u][k] = u < nb ? c[u * cqs + k * cks] : 0.0;
+    if (t->v) {
+        double vr[FSX_QB][6];
+        const double *v = t->v + i0 * t->vqs;
+        const long vks = t->vks, vqs = t->vqs;
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+        for (int u = 0; u < FSX_QB; u++)
+            for (int k = 0; k < 6; k++) vr[u][k] = u < nb ? v[u * vqs + k * vks] : 0.0;
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+        for (int u = 0; u < FSX_QB; u++)
+            for (int k = 0; k < 6; k++) {
+                double x = cr[u][k];
+                x = x + vr[u][k];
+                blk[u][k] = x * t->scale;
+            }
+    } else {
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+        for (int u = 0; u < FSX_QB; u++)
+            for (int k = 0; k < 6; k++) blk[u][k] = cr[u][k] * t->scale;
+    }
+}
+
 /* I(q) at G = c1^3 exp(-mult (c1^2 - 1) q^2) and c2 (min_saxs.c:285-287) */
 #define FSX_IN(v, G, c2)                                                              \
     ((v)[0] - (G) * (v)[1] + (c2) * (v)[2] + (G) * (G) * (v)[3] - (G) * (c2) * (v)[4] \
@@ -116,31 +156,44 @@ FSX_HD double fsx_in(const fsx_ct *t, int i, double G, double c2)
     return FSX_IN(v, G, c2);
 }
 
+/* The q loops below fetch the profile FSX_QB q values at a time (all loads
+ * of a block in flight together), then run the steps of the block in order;
+ * the arithmetic and its order are those of the reference loops. */
+
 /* min_saxs.c:261-319 */
 FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2)
 {
     const double *a = p->a, *q = p->q;
-    double cur[6], nxt[6];
-    fsx_ct_get6(t, 0, cur);
+    const int qnum = p->qnum;
+    double v0[6];
+    fsx_ct_get6(t, 0, v0);
     double corr = -p->mult * (c1 * c1 - 1.0);
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
-    double in_prev = FSX_IN(cur, G, c2);
+    double in_prev = FSX_IN(v0, G, c2);
     double c1c = c1 * c1 * c1;
     double q_prev = -1.0, up = 0.0, down = 0.0;
-    for (int i = 0; i < p->qnum; i++) {
-        if (i + 1 < p->qnum) fsx_ct_get6(t, i + 1, nxt);
-        double qc = q[i];
-        G = c1c * exp(corr * qc * qc);
-        double in = FSX_IN(cur, G, c2);
-        double tan_ = (in - in_prev) / (qc - q_prev);
-        double buf = in - tan_ * qc;
-        up += buf * a[i * 6 + 1] + tan_ * a[i * 6 + 2];
-        down += buf * buf * a[i * 6 + 3] + 2.0 * tan_ * buf * a[i * 6 + 4]
-              + tan_ * tan_ * a[i * 6 + 5];
-        in_prev = in;
-        q_prev = qc;
-        if (i + 1 < p->qnum)
-            for (int k = 0; k < 6; k++) cur[k] = nxt[k];
+    for (int i0 = 0; i0 < qnum; i0 += FSX_QB) {
+        double blk[FSX_QB][6];
+        fsx_ct_block(t, i0, qnum - i0 < FSX_QB ? qnum - i0 : FSX_QB, blk);
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+        for (int u = 0; u < FSX_QB; u++) {
+            const int i = i0 + u;
+            if (i < qnum) {
+                const double *cur = blk[u];
+                double qc = q[i];
+                G = c1c * exp(corr * qc * qc);
+                double in = FSX_IN(cur, G, c2);
+                double tan_ = (in - in_prev) / (qc - q_prev);
+                double buf = in - tan_ * qc;
+                up += buf * a[i * 6 + 1] + tan_ * a[i * 6 + 2];
+                down += buf * buf * a[i * 6 + 3] + 2.0 * tan_ * buf * a[i * 6 + 4]
+                      + tan_ * tan_ * a[i * 6 + 5];
+                in_prev = in;
+                q_prev = qc;
+            }
+        }
     }
     return up / down;
 }
@@ -150,50 +203,59 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
                           const double *x, double *g, double *fval)
 {
     const double *a = p->a, *q = p->q;
-    double cur[6], nxt[6];
-    fsx_ct_get6(t, 0, cur);
+    const int qnum = p->qnum;
+    double v0[6];
+    fsx_ct_get6(t, 0, v0);
     double c1 = x[0], c2 = x[1];
     double g0 = 0.0, g1 = 0.0, mult = p->mult;
     double corr = -mult * (c1 * c1 - 1.0);
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
     double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * q[0] * q[0]);
-    double in_prev = FSX_IN(cur, G, c2);
-    double d1p = -Gd * cur[1] + 2.0 * G * Gd * cur[3] - Gd * c2 * cur[4];
-    double d2p = cur[2] - G * cur[4] + 2.0 * c2 * cur[5];
+    double in_prev = FSX_IN(v0, G, c2);
+    double d1p = -Gd * v0[1] + 2.0 * G * Gd * v0[3] - Gd * c2 * v0[4];
+    double d2p = v0[2] - G * v0[4] + 2.0 * c2 * v0[5];
     double q_prev = -1.0, score = 0.0, c1c = c1 * c1 * c1;
-    for (int i = 0; i < p->qnum; i++) {
-        if (i + 1 < p->qnum) fsx_ct_get6(t, i + 1, nxt);
-        double qc = q[i];
-        G = c1c * exp(corr * qc * qc);
-        Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
-        double vd = cur[1], vw = cur[2], dd = cur[3];
-        double dw = cur[4], ww = cur[5];
-        double in = cur[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww;
-        double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);
-        double d2 = vw - G * dw + 2.0 * c2 * ww;
-        double buf = 1.0 / (qc - q_prev);
-        double tan_ = (in - in_prev) * buf;
-        double t1 = (d1 - d1p) * buf;
-        double t2 = (d2 - d2p) * buf;
-        const double *ai = a + i * 6;
-        g0 += 2.0 * k * (-(d1 - t1 * qc) * ai[1] - t1 * ai[2]
-                         + k * ((in - tan_ * qc) * (d1 - t1 * qc) * ai[3]
-                                + (in * t1 + d1 * tan_ - 2.0 * tan_ * t1 * qc) * ai[4]
-                                + tan_ * t1 * ai[5]));
-        g1 += 2.0 * k * (-(d2 - t2 * qc) * ai[1] - t2 * ai[2]
-                         + k * ((in - tan_ * qc) * (d2 - t2 * qc) * ai[3]
-                                + (in * t2 + d2 * tan_ - 2.0 * tan_ * t2 * qc) * ai[4]
-                                + tan_ * t2 * ai[5]));
-        buf = in - tan_ * qc;
-        score += ai[0] + k * (-2.0 * buf * ai[1] - 2.0 * tan_ * ai[2]
-                              + k * (buf * buf * ai[3] + 2.0 * buf * tan_ * ai[4]
-                                     + tan_ * tan_ * ai[5]));
-        in_prev = in;
-        d1p = d1;
-        d2p = d2;
-        q_prev = qc;
-        if (i + 1 < p->qnum)
-            for (int j = 0; j < 6; j++) cur[j] = nxt[j];
+    for (int i0 = 0; i0 < qnum; i0 += FSX_QB) {
+        double blk[FSX_QB][6];
+        fsx_ct_block(t, i0, qnum - i0 < FSX_QB ? qnum - i0 : FSX_QB, blk);
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+        for (int u = 0; u < FSX_QB; u++) {
+            const int i = i0 + u;
+            if (i < qnum) {
+                const double *cur = blk[u];
+                double qc = q[i];
+                G = c1c * exp(corr * qc * qc);
+                Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
+                double vd = cur[1], vw = cur[2], dd = cur[3];
+                double dw = cur[4], ww = cur[5];
+                double in = cur[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww;
+                double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);
+                double d2 = vw - G * dw + 2.0 * c2 * ww;
+                double buf = 1.0 / (qc - q_prev);
+                double tan_ = (in - in_prev) * buf;
+                double t1 = (d1 - d1p) * buf;
+                double t2 = (d2 - d2p) * buf;
+                const double *ai = a + i * 6;
+                g0 += 2.0 * k * (-(d1 - t1 * qc) * ai[1] - t1 * ai[2]
+                                 + k * ((in - tan_ * qc) * (d1 - t1 * qc) * ai[3]
+                                        + (in * t1 + d1 * tan_ - 2.0 * tan_ * t1 * qc) * ai[4]
+                                        + tan_ * t1 * ai[5]));
+                g1 += 2.0 * k * (-(d2 - t2 * qc) * ai[1] - t2 * ai[2]
+                                 + k * ((in - tan_ * qc) * (d2 - t2 * qc) * ai[3]
+                                        + (in * t2 + d2 * tan_ - 2.0 * tan_ * t2 * qc) * ai[4]
+                                        + tan_ * t2 * ai[5]));
+                buf = in - tan_ * qc;
+                score += ai[0] + k * (-2.0 * buf * ai[1] - 2.0 * tan_ * ai[2]
+                                      + k * (buf * buf * ai[3] + 2.0 * buf * tan_ * ai[4]
+                                             + tan_ * tan_ * ai[5]));
+                in_prev = in;
+                d1p = d1;
+                d2p = d2;
+                q_prev = qc;
+            }
+        }
     }
     g[0] = g0;
     g[1] = g1;

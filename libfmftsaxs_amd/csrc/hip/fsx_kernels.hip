@@ -34,6 +34,7 @@
 #define FSX_NXCD 8
 
 struct cplx { double re, im; };
+struct cplx32 { float re, im; };   /* element of the mixed-precision (FP32) tables */
 
 /* ---------------------------------------------------------------------- */
 /* device helpers */
@@ -132,7 +133,8 @@ __host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
  * so the mirrored entries are bitwise equal, as in fftsaxs.c:213-238.  One
  * thread per (z, q, am, l, l1) of the full cube; valid entries are stored in
  * the packed layout. */
-__global__ void k_tmat(cplx *T, const double *bes, const double *ds, int znum, int Q, int L)
+template <typename E>
+__global__ void k_tmat(E *T, const double *bes, const double *ds, int znum, int Q, int L)
 {
 #pragma clang fp contract(off)
     const int K = L + 1, N = 2 * L + 1;
@@ -154,7 +156,7 @@ __global__ void k_tmat(cplx *T, const double *bes, const double *ds, int znum, i
         re += ipr[p % 4] * v;
         im += ipi[p % 4] * v;
     }
-    cplx out;
+    E out;
     out.re = re;
     out.im = im;
     T[zq * tpk_sq(K) + tpk_row(m, l, K) + (l1 - m)] = out;
@@ -165,6 +167,15 @@ __device__ __forceinline__ double wave_sum(double v)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FSX_WAVE);
     return v;
+}
+
+/* value of v in lane k (k wave-uniform), through two scalar reads */
+__device__ __forceinline__ double lane_bcast(double v, int k)
+{
+    int2 w = *reinterpret_cast<int2 *>(&v);
+    w.x = __builtin_amdgcn_readlane(w.x, k);
+    w.y = __builtin_amdgcn_readlane(w.y, k);
+    return *reinterpret_cast<double *>(&w);
 }
 
 /* x mod n in [0, n) for |x| < 2^24, n >= 1: a float quotient estimate
@@ -207,40 +218,68 @@ __global__ void k_twiddle(cplx *dst, const double *src_re, const double *src_im,
     dst[i] = v;
 }
 
+/* one block per (q, X, l, gamma chunk of gc): the 2l+1 source rows of l,
+ * gamma in the chunk, are staged in LDS ([m1][gamma]); each wavefront then
+ * forms out^X_{lm}(beta, gamma) for (beta, m) pairs, lanes over gamma. */
+template <typename E>
 __global__ void __launch_bounds__(256)
-k_rotate(cplx *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
-         int nbeta, int ng, int Q, int L)
+k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
+         int nbeta, int ng, int Q, int L, int gc)
 {
-    const int K2 = (L + 1) * (L + 1), N = 2 * L + 1;
-    const int *ent_l = ent, *ent_m = ent + K2, *ent_c = ent + 2 * K2, *ent_ec = ent + 3 * K2;
-    const int *csize = ent + 4 * K2;
+#pragma clang fp contract(fast)
+    extern __shared__ cplx sR[];   /* [2l+1][gc] */
+    const int K = L + 1, K2 = K * K, N = 2 * L + 1;
+    const int *ent_c = ent + 2 * K2, *ent_ec = ent + 3 * K2, *csize = ent + 4 * K2;
+    const int *mstart = ent + 5 * K2;
     const int nrows = nbeta * ng;
     const int lane = threadIdx.x % FSX_WAVE, wave = threadIdx.x / FSX_WAVE;
     const int nwave = blockDim.x / FSX_WAVE;
-    /* block = (beta, q, X) */
-    const int X = blockIdx.x % 3;
-    const int q = (blockIdx.x / 3) % Q;
-    const int b = blockIdx.x / (3 * Q);
-    const double *d = dW + (long)b * (L + 1) * N * N;
-    const cplx *sq = srct + ((long)q * 3 + X) * K2 * ng;
-    for (int e = wave; e < K2; e += nwave) {
-        const int l = ent_l[e], m = ent_m[e];
-        const double *drow = d + ((long)l * N + (L + m)) * N + L;
-        const cplx *a = sq + (long)(l * (l + 1)) * ng;
+    const int nch = (ng + gc - 1) / gc;
+    int r = blockIdx.x;
+    const int ch = r % nch; r /= nch;
+    const int l = r % K; r /= K;
+    const int X = r % 3;
+    const int q = r / 3;
+    const int g0 = ch * gc, gn = (ng - g0) < gc ? (ng - g0) : gc;
+    const int w = 2 * l + 1;
+    const cplx *src = srct + (((long)q * 3 + X) * K2 + l * (l + 1) - l) * ng + g0;
+    for (int i = threadIdx.x; i < w * gc; i += blockDim.x) {
+        int m1 = i / gc, g = i - m1 * gc;
+        if (g < gn) sR[i] = src[(long)m1 * ng + g];
+    }
+    __syncthreads();
+    for (int t = wave; t < nbeta * w; t += nwave) {
+        const int b = t / w, m = t - b * w - l;
+        const double *drow = dW + (long)b * K * N * N + ((long)l * N + (L + m)) * N + (L - l);
+        /* the row d^l_{m, -l..l}(beta): one element per lane, one load, then
+         * broadcast term by term (w <= 2L+1 <= 81 lanes over two registers) */
+        const double dk0 = lane < w ? drow[lane] : 0.0;
+        const double dk1 = lane + FSX_WAVE < w ? drow[lane + FSX_WAVE] : 0.0;
+        /* entry (l, m) in m-major order */
+        const int e = mstart[m + L] + (l - (m < 0 ? -m : m));
         const int c = ent_c[e];
-        cplx *o = out + coff[c] + ent_ec[e];
-        for (int g = lane; g < ng; g += FSX_WAVE) {
+        E *o = out + coff[c] + ent_ec[e];
+        const long cs = csize[c];
+        /* every lane runs the loop (the broadcasts need the whole
+         * wavefront); lanes past the chunk's end compute nothing */
+        for (int gw = 0; gw < gn; gw += FSX_WAVE) {
+            const int g = gw + lane;
+            const bool on = g < gn;
             double re = 0.0, im = 0.0;
-            for (int m1 = -l; m1 <= l; m1++) {
-                double dv = drow[m1];
-                cplx t = a[(long)m1 * ng + g];
-                re += dv * t.re;
-                im += dv * t.im;
+            for (int k = 0; k < w; k++) {
+                double dv = k < FSX_WAVE ? lane_bcast(dk0, k) : lane_bcast(dk1, k - FSX_WAVE);
+                if (on) {
+                    cplx v = sR[k * gc + g];
+                    re += dv * v.re;
+                    im += dv * v.im;
+                }
             }
-            cplx v;
-            v.re = re;
-            v.im = im;
-            o[(((long)q * nrows + (long)b * ng + g) * 3 + X) * csize[c]] = v;
+            if (on) {
+                E v;
+                v.re = re;
+                v.im = im;
+                o[(((long)q * nrows + (long)b * ng + g0 + g) * 3 + X) * cs] = v;
+            }
         }
     }
 }
@@ -352,22 +391,26 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  *   2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e])
  * over the chunk's entries e = (l, m), stored (first chunk) or added into
  * acc[(s*Q + q)*6 + k]. */
-template <int J, int TST>
+template <int J, int TST, typename E>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK)
-k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double *twa,
+k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int na, int ng,
         int zn, int first)
 {
-    extern __shared__ cplx smem[];
-    cplx *sB = smem;            /* [3][nc] */
-    cplx *sU = smem + 3 * nc;   /* [3][nc] */
-    cplx *sT = smem + 6 * nc;   /* packed T block (TST) */
-    const int K = L + 1, K2 = K * K, Tn = tpk_sq(K);
+#pragma clang fp contract(fast)
+    typedef decltype(E::re) F;   /* arithmetic type of the contraction */
+    extern __shared__ double smem_raw[];
+    E *smem = reinterpret_cast<E *>(smem_raw);
+    E *sB = smem;            /* [3][nc] */
+    E *sU = smem + 3 * nc;   /* [3][nc] */
+    E *sT = smem + 6 * nc;   /* T block, transposed per |m| (TST) */
+    const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
+    const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
     const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
-    const float inv_na = 1.0f / (float)na;
+    (void)na;
     /* per-thread U entry, per-lane pairing entries */
     const bool uact = tid < nc;
     int ul = 0, um = 0, uam = 0;
@@ -377,7 +420,11 @@ k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double
         uam = um < 0 ? -um : um;
     }
     const int ub0 = uact ? mstart[um + L] - cstart : 0;
-    const int urow = uact ? tpk_row(uam, ul, K) : 0;
+    /* T in LDS is stored per |m| block as [l1][l] (consecutive l, i.e.
+     * consecutive threads, at consecutive addresses): element (am, l, l1)
+     * at tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am) */
+    const int urow = uact ? (TST ? tpk_row(uam, uam, K) + (ul - uam) : tpk_row(uam, ul, K)) : 0;
+    const int ustep = TST ? K - uam : 1;
     int em[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -390,29 +437,30 @@ k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int q = (int)(tile / nbg), bg = (int)(tile - (long)q * nbg);
         const int b2 = bg / ng, g2 = bg - (bg / ng) * ng;
-        const cplx *Bq = Bc + (((long)q * nrows + bg) * 3) * nc;
+        const E *Bq = Bc + (((long)q * nrows + bg) * 3) * nc;
         bool staged = false;
         for (int z = 0; z < zn; z++) {
             const int key = (z * nbeta + b2) * ng + g2;
             const int cnt = counts[key];
             if (cnt == 0) continue;
             const int start = offs[key];
-            const cplx *Tq = T + ((long)z * Q + q) * Tn;
+            const E *Tq = T + ((long)z * Q + q) * Tn;
             /* pose loader: receptor row entries, twiddles, accumulator */
-            cplx rr[3][J];
-            double wr[J], wi[J], old = 0.0;
+            E rr[3][J];
+            F wr[J], wi[J];
+            double old = 0.0;
             int s = 0;
             auto load_pose = [&](int p) {
                 s = start + p;
                 int4 rec = recs[s];
-                const cplx *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
+                const E *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
                     if (ec < nc) {
-                        int kk = mod_small(em[j] * rec.z, na, inv_na);
-                        wr[j] = twa[2 * kk];
-                        wi[j] = twa[2 * kk + 1];
+                        const double *tw = tw2 + 2 * ((long)rec.z * N + em[j] + L);
+                        wr[j] = (F)tw[0];
+                        wi[j] = (F)tw[1];
 #pragma unroll
                         for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + ec];
                     }
@@ -420,23 +468,49 @@ k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double
                 old = (!first && writer) ? acc[((long)s * Q + q) * 6 + slot] : 0.0;
             };
             if (wave < cnt) load_pose(wave);
+            /* staging: four loads per thread in flight (an index past the end
+             * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
-                for (int i = tid; i < 3 * nc; i += FSX_CROSS_BLOCK) sB[i] = Bq[i];
+                const int nb3 = 3 * nc;
+                for (int b0 = tid; b0 < nb3; b0 += 4 * FSX_CROSS_BLOCK) {
+                    const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
+                    const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
+                    E t0 = Bq[b0];
+                    E t1 = Bq[i1 < nb3 ? i1 : b0];
+                    E t2 = Bq[i2 < nb3 ? i2 : b0];
+                    E t3 = Bq[i3 < nb3 ? i3 : b0];
+                    sB[b0] = t0;
+                    if (i1 < nb3) sB[i1] = t1;
+                    if (i2 < nb3) sB[i2] = t2;
+                    if (i3 < nb3) sB[i3] = t3;
+                }
                 staged = true;
             }
-            if (TST)
-                for (int i = tid; i < Tn; i += FSX_CROSS_BLOCK) sT[i] = Tq[i];
+            if (TST) {
+                for (int b0 = tid; b0 < Tn; b0 += 4 * FSX_CROSS_BLOCK) {
+                    const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
+                    const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
+                    const int j1 = i1 < Tn ? i1 : b0, j2 = i2 < Tn ? i2 : b0;
+                    const int j3 = i3 < Tn ? i3 : b0;
+                    E t0 = Tq[b0], t1 = Tq[j1], t2 = Tq[j2], t3 = Tq[j3];
+                    int d0 = tmap[b0], d1 = tmap[j1], d2 = tmap[j2], d3 = tmap[j3];
+                    sT[d0] = t0;
+                    if (i1 < Tn) sT[d1] = t1;
+                    if (i2 < Tn) sT[d2] = t2;
+                    if (i3 < Tn) sT[d3] = t3;
+                }
+            }
             __syncthreads();
             if (uact) {
-                const cplx *Tr = (TST ? sT : Tq) + urow;
-                const cplx *bp = sB + ub0;
+                const E *Tr = (TST ? sT : Tq) + urow;
+                const E *bp = sB + ub0;
                 const int nk = L - uam + 1;
-                double ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+                F ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
                 for (int k = 0; k < nk; k++) {
-                    cplx tv = Tr[k];
+                    E tv = Tr[k * ustep];
 #pragma unroll
                     for (int Y = 0; Y < 3; Y++) {
-                        cplx bv = bp[Y * nc + k];
+                        E bv = bp[Y * nc + k];
                         /* conj(t) * b */
                         ur[Y] += tv.re * bv.re + tv.im * bv.im;
                         ui[Y] += tv.re * bv.im - tv.im * bv.re;
@@ -451,26 +525,28 @@ k_cross(double *acc, const cplx *Rc, const cplx *Bc, const cplx *T, const double
             __syncthreads();
             for (int p = wave; p < cnt; p += nwave) {
                 if (p != wave) load_pose(p);
-                double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
                     if (ec < nc) {
-                        cplx rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
-                        cplx uv = sU[ec], ud = sU[nc + ec], uw = sU[2 * nc + ec];
-                        double w_r = wr[j], w_i = wi[j];
+                        E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
+                        E uv = sU[ec], ud = sU[nc + ec], uw = sU[2 * nc + ec];
+                        F w_r = wr[j], w_i = wi[j];
                         /* twiddled receptor factor: w * R^X */
-                        double vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
-                        double dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
-                        double xr = w_r * rw.re - w_i * rw.im, xi = w_r * rw.im + w_i * rw.re;
-                        v[0] += vr * uv.re - vi * uv.im;
-                        v[1] += (vr * ud.re - vi * ud.im) + (dr * uv.re - di * uv.im);
-                        v[2] += (vr * uw.re - vi * uw.im) + (xr * uv.re - xi * uv.im);
-                        v[3] += dr * ud.re - di * ud.im;
-                        v[4] += (dr * uw.re - di * uw.im) + (xr * ud.re - xi * ud.im);
-                        v[5] += xr * uw.re - xi * uw.im;
+                        F vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
+                        F dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
+                        F xr = w_r * rw.re - w_i * rw.im, xi = w_r * rw.im + w_i * rw.re;
+                        va[0] += vr * uv.re - vi * uv.im;
+                        va[1] += (vr * ud.re - vi * ud.im) + (dr * uv.re - di * uv.im);
+                        va[2] += (vr * uw.re - vi * uw.im) + (xr * uv.re - xi * uv.im);
+                        va[3] += dr * ud.re - di * ud.im;
+                        va[4] += (dr * uw.re - di * uw.im) + (xr * ud.re - xi * ud.im);
+                        va[5] += xr * uw.re - xi * uw.im;
                     }
                 }
+                /* the cross-lane sums run in FP64 */
+                double v[8] = {va[0], va[1], va[2], va[3], va[4], va[5], 0.0, 0.0};
                 /* transpose-reduce over the 64 lanes: after the three halving
                  * steps lane bits 5,4,3 select the slot, then a plain sum over
                  * bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
@@ -669,7 +745,7 @@ static hipStream_t g_stream;
 enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
-    B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
+    B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
     B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_NBUF
 };
 static void *g_buf[B_NBUF];
@@ -752,6 +828,59 @@ static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t);
         if (r_ != FSX_OK) return r_;   \
     } while (0)
 
+/* launches of the element-type dependent stages (FP64 or FP32 tables) */
+template <typename E>
+static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
+{
+    hipLaunchKernelGGL(k_tmat<E>, dim3(nblk(nt, 256)), dim3(256), 0, s, (E *)T,
+                       (const double *)g_buf[B_BES], (const double *)g_buf[B_DS], zn, Q, L);
+}
+
+template <typename E>
+static void launch_rotate(hipStream_t s, void *out, const cplx *srct, long nrot, size_t lds,
+                          int nb, int ng, int Q, int L, int gc)
+{
+    hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)out, srct,
+                       (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
+                       (const long *)g_buf[B_COFF], nb, ng, Q, L, gc);
+}
+
+template <typename E>
+static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
+                         int Tn, long grid, long ntiles, int nrows, int Q, int L, int nb, int na,
+                         int ng, int zn)
+{
+    const int tst = (long)Tn * (long)sizeof(E) <= 40 * 1024;
+    for (int c = 0; c < C; c++) {
+        if (csz[c] == 0) continue;
+        int jn = (csz[c] + FSX_WAVE - 1) / FSX_WAVE;
+        void (*kern)(double *, const E *, const E *, const E *, const double *, const int *,
+                     int, int, const int *, const int *, const int4 *, long, int, int, int,
+                     int, int, int, int, int);
+        if (tst) kern = jn <= 1 ? k_cross<1, 1, E> : jn <= 2 ? k_cross<2, 1, E> : k_cross<4, 1, E>;
+        else kern = jn <= 1 ? k_cross<1, 0, E> : jn <= 2 ? k_cross<2, 0, E> : k_cross<4, 0, E>;
+        size_t lds = sizeof(E) * (6 * (size_t)csz[c] + (tst ? Tn : 0));
+        /* persistent-style grid: the blocks that fit the chip at once
+         * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
+        long g = grid;
+        if (g <= 0) {
+            int per = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, FSX_CROSS_BLOCK, lds)
+                    != hipSuccess || per < 1)
+                per = 2;
+            g = (long)g_ncu * per;
+        }
+        if (g > ntiles) g = ntiles;
+        hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(FSX_CROSS_BLOCK), lds, s,
+                           (double *)g_buf[B_ACC], (const E *)g_buf[B_RT] + coff[c],
+                           (const E *)g_buf[B_BT] + coff[c], (const E *)g_buf[B_T],
+                           (const double *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
+                           (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
+                           (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, na, ng, zn,
+                           c == 0);
+    }
+}
+
 /* Entry tables of the m-major coefficient order used by the rotated tables
  * and the cross-term kernel: e runs over m = -L..L, then l = |m|..L.  The
  * entries are cut into C chunks of whole m values with about K2/C entries
@@ -797,11 +926,15 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     if ((long)L * na >= (1L << 24) || (long)L * ng >= (1L << 24)) return FSX_ERR_ARG;
     const size_t dsz = (size_t)K * N * N;
     const size_t spf = (size_t)3 * Q * K2;
+    const int mixed = (A->flags & FSX_MIXED_FP32) != 0;
+    const size_t es = mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
     hipStream_t s = g_stream;
     /* m-chunks: one launch per chunk of entries; one chunk unless a chunk
      * would exceed 4 entries per lane (FSX_CROSS_CHUNKS overrides) */
     int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
-    int *ent = (int *)malloc(sizeof(int) * (7 * K2 + 2));
+    const int Tn = tpk_sq(K);
+    const size_t nent = 7 * (size_t)K2 + Tn + 2;
+    int *ent = (int *)malloc(sizeof(int) * nent);
     long *coff = (long *)malloc(sizeof(long) * (N + 1));
     if (!ent || !coff) {
         free(ent);
@@ -823,6 +956,29 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         free(coff);
         return FSX_ERR_ARG;
     }
+    /* packed T block index -> its place in the transposed LDS copy */
+    {
+        int *tmap = ent + 7 * K2, i = 0;
+        for (int am = 0; am <= L; am++)
+            for (int l = am; l <= L; l++)
+                for (int l1 = am; l1 <= L; l1++)
+                    tmap[i++] = tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am);
+    }
+    /* twiddle e^{-i m a2 2pi/na} per (a2, m): the same table entries the
+     * reference's exact transform uses, gathered by (m a2) mod na */
+    double *tw2 = (double *)malloc(sizeof(double) * 2 * (size_t)na * N);
+    if (!tw2) {
+        free(ent);
+        free(coff);
+        return FSX_ERR_NOMEM;
+    }
+    for (int a2 = 0; a2 < na; a2++)
+        for (int m = -L; m <= L; m++) {
+            long k = ((long)m * a2) % na;
+            if (k < 0) k += na;
+            tw2[2 * ((long)a2 * N + m + L)] = A->tw_a[2 * k];
+            tw2[2 * ((long)a2 * N + m + L) + 1] = A->tw_a[2 * k + 1];
+        }
     const int *csz = ent + 4 * K2, *cst = ent + 6 * K2;
     coff[0] = 0;
     for (int c = 0; c < C; c++) coff[c + 1] = coff[c] + (long)Q * nrows * 3 * csz[c];
@@ -844,10 +1000,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     } while (0)
     {
     RCF(ensure(B_BES, sizeof(double) * zn * Q * N));
-    const int Tn = tpk_sq(K);
-    RCF(ensure(B_T, sizeof(cplx) * zn * Q * (size_t)Tn));
-    RCF(ensure(B_RT, sizeof(cplx) * (size_t)nrows * spf));
-    RCF(ensure(B_BT, sizeof(cplx) * (size_t)nrows * spf));
+    RCF(ensure(B_T, es * zn * Q * (size_t)Tn));
+    RCF(ensure(B_RT, es * (size_t)nrows * spf));
+    RCF(ensure(B_BT, es * (size_t)nrows * spf));
     RCF(ensure(B_A, sizeof(double) * 2 * spf));
     RCF(ensure(B_B, sizeof(double) * 2 * spf));
     RCF(ensure(B_DW, sizeof(double) * dsz * nb));
@@ -865,7 +1020,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     RCF(ensure(B_FILL, sizeof(int) * nkeys));
     RCF(ensure(B_RECS, sizeof(int4) * n));
     RCF(ensure(B_ACC, sizeof(double) * 6 * Q * (size_t)n));
-    RCF(ensure(B_ENT, sizeof(int) * (7 * K2 + 2)));
+    RCF(ensure(B_ENT, sizeof(int) * nent));
+    RCF(ensure(B_TW2, sizeof(double) * 2 * (size_t)na * N));
     RCF(ensure(B_COFF, sizeof(long) * (C + 1)));
     RCF(ensure(B_NEXT, sizeof(int)));
     RCF(ensure(B_TALLY, 2 * sizeof(unsigned long long)));
@@ -883,7 +1039,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipMemcpyAsync(g_buf[B_Z], A->zvals, sizeof(double) * zn, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(g_buf[B_AM], A->a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(g_buf[B_CINT], A->cint, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_ENT], ent, sizeof(int) * (7 * K2 + 2), hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_ENT], ent, sizeof(int) * nent, hipMemcpyHostToDevice, s));
+    HIPF(hipMemcpyAsync(g_buf[B_TW2], tw2, sizeof(double) * 2 * (size_t)na * N, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(g_buf[B_COFF], coff, sizeof(long) * (C + 1), hipMemcpyHostToDevice, s));
 
     const long long *dposes;
@@ -923,26 +1080,29 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g_buf[B_BES],
                        (const double *)g_buf[B_Z], (const double *)g_buf[B_Q], zn, Q, N);
     long nt = (long)zn * Q * K * K2;
-    hipLaunchKernelGGL(k_tmat, dim3(nblk(nt, 256)), dim3(256), 0, s, (cplx *)g_buf[B_T],
-                       (const double *)g_buf[B_BES], (const double *)g_buf[B_DS], zn, Q, L);
+    if (mixed) launch_tmat<cplx32>(s, g_buf[B_T], nt, zn, Q, L);
+    else launch_tmat<cplx>(s, g_buf[B_T], nt, zn, Q, L);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[1], s));
     /* rotated receptor and ligand coefficients, m-chunked */
     {
         long ntw = (long)Q * 3 * K2 * ng;
         cplx *srct = (cplx *)g_buf[B_SRCT];
+        /* gamma chunk: the staged rows (2L+1) x gc complex stay within 48 KB */
+        int gc = ng < FSX_WAVE ? ng : FSX_WAVE;
+        while (gc > 1 && (size_t)N * gc * sizeof(cplx) > 48 * 1024) gc /= 2;
+        const long nrot = (long)Q * 3 * K * ((ng + gc - 1) / gc);
+        const size_t lds_rot = (size_t)N * gc * sizeof(cplx);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dA, (const double *)(dA + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
-        hipLaunchKernelGGL(k_rotate, dim3((unsigned)(nb * Q * 3)), dim3(256), 0, s,
-                           (cplx *)g_buf[B_RT], (const cplx *)srct, (const double *)g_buf[B_DW],
-                           (const int *)g_buf[B_ENT], (const long *)g_buf[B_COFF], nb, ng, Q, L);
+        if (mixed) launch_rotate<cplx32>(s, g_buf[B_RT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
+        else launch_rotate<cplx>(s, g_buf[B_RT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
-        hipLaunchKernelGGL(k_rotate, dim3((unsigned)(nb * Q * 3)), dim3(256), 0, s,
-                           (cplx *)g_buf[B_BT], (const cplx *)srct, (const double *)g_buf[B_DW],
-                           (const int *)g_buf[B_ENT], (const long *)g_buf[B_COFF], nb, ng, Q, L);
+        if (mixed) launch_rotate<cplx32>(s, g_buf[B_BT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
+        else launch_rotate<cplx>(s, g_buf[B_BT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
@@ -960,25 +1120,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipEventRecord(ev[3], s));
     /* per-pose cross terms, one launch per m-chunk, accumulated in order */
     const long ntiles = (long)Q * nb * ng;
-    const int tst = (long)Tn * (long)sizeof(cplx) <= 40 * 1024;
-    long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 3, 1, 16);
-    if (grid > ntiles) grid = ntiles;
-    for (int c = 0; c < C; c++) {
-        if (csz[c] == 0) continue;
-        int jn = (csz[c] + FSX_WAVE - 1) / FSX_WAVE;
-        void (*kern)(double *, const cplx *, const cplx *, const cplx *, const double *,
-                     const int *, int, int, const int *, const int *, const int4 *, long, int,
-                     int, int, int, int, int, int, int);
-        if (tst) kern = jn <= 1 ? k_cross<1, 1> : jn <= 2 ? k_cross<2, 1> : k_cross<4, 1>;
-        else kern = jn <= 1 ? k_cross<1, 0> : jn <= 2 ? k_cross<2, 0> : k_cross<4, 0>;
-        size_t lds = sizeof(cplx) * (6 * (size_t)csz[c] + (tst ? Tn : 0));
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(FSX_CROSS_BLOCK), lds, s,
-                           (double *)g_buf[B_ACC], (const cplx *)g_buf[B_RT] + coff[c],
-                           (const cplx *)g_buf[B_BT] + coff[c], (const cplx *)g_buf[B_T],
-                           (const double *)g_buf[B_TWA], (const int *)g_buf[B_ENT], cst[c], csz[c],
-                           (const int *)counts, (const int *)offs, (const int4 *)g_buf[B_RECS],
-                           ntiles, nrows, Q, L, nb, na, ng, zn, c == 0);
-    }
+    long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
+    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, ntiles, nrows, Q, L, nb, na, ng, zn);
+    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, ntiles, nrows, Q, L, nb, na, ng, zn);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue */
@@ -1037,6 +1181,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
 out:
     free(ent);
     free(coff);
+    free(tw2);
 #undef RCF
 #undef HIPF
     return rc;

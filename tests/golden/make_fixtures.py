@@ -12,6 +12,8 @@ suite, tests/saxs_test.c):
   ref_fitted_profile.txt fitted profile of the native dimer (saxs_test.c:365-407)
   euler_z40.txt          the Euler-list lines of euler_coords.000.00 with z = 40
   ref_spf.npz            receptor SPF coefficients written by the reference (ref_spf)
+  pdb_formfactor_mapping_clean.prm  the (residue, atom) -> form-factor type mapping
+                         file the reference's tests load (saxs_test.c:74)
   atoms_<name>.npz       atom tables of the test PDB files: coordinates, residue and
                          atom names, vdW radius (atoms.0.0.6.prm...), vacuum and
                          dummy zero-angle form factors of the mapped type
@@ -86,7 +88,8 @@ def main(ref="/root/reference"):
     data = os.path.join(ref, "tests", "data")
     for src, dst in [("ref_chi", "ref_chi.txt"), ("ref_saxs", "ref_saxs.txt"),
                      ("ref_profile", "ref_profile.txt"),
-                     ("ref_fitted_profile", "ref_fitted_profile.txt")]:
+                     ("ref_fitted_profile", "ref_fitted_profile.txt"),
+                     ("pdb_formfactor_mapping_clean.prm", "pdb_formfactor_mapping_clean.prm")]:
         shutil.copyfile(os.path.join(data, src), os.path.join(HERE, dst))
     with open(os.path.join(data, "euler_coords.000.00")) as f, \
             open(os.path.join(HERE, "euler_z40.txt"), "w") as g:
