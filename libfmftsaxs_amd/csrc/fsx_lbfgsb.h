@@ -4,8 +4,9 @@
  *
  * One implementation compiled twice: as plain C11 for the host entry points
  * (sxs_lbfgs_fitting, sxs_fit_params) and as HIP device code for the fused
- * per-pose fit kernel.  State lives in a fixed-size struct (no allocation),
- * so a GPU thread owns one fit.
+ * per-pose fit kernel.  State lives in a fixed-size struct (no allocation)
+ * whose every subscript is a compile-time constant after unrolling, so a
+ * GPU thread owns one fit and keeps its state in registers.
  *
  * Semantics follow the reference:
  *   objective    min_saxs.c:3-105 (gradient), :261-319 (sxs_best_scale)
@@ -30,7 +31,9 @@
 #include <float.h>
 
 #ifdef __HIPCC__
-#define FSX_HD __host__ __device__ static inline
+/* inlined before any of these bodies is optimised on its own, so that the
+ * optimiser sees the fit state as one local object of the kernel */
+#define FSX_HD __host__ __device__ static inline __attribute__((always_inline))
 #else
 #define FSX_HD static inline
 #endif
@@ -263,258 +266,428 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
 }
 
 /* ---------------------------------------------------------------------- */
-/* L-BFGS-B state for n = 2, m = 3; matrices column-major, 1-based macros. */
+/* L-BFGS-B for n = 2 variables and m = 3 correction pairs, the sizes the
+ * wrapper always uses (min_saxs.c:196-259).  Every subscript into the state
+ * is a compile-time constant once the fixed-bound loops below are unrolled,
+ * so a GPU thread keeps the whole state in registers:
+ *  - the correction pairs S (ws) and Y (wy) are stored oldest first: column
+ *    j is the reference's ring-buffer column ((head + j - 2) mod m) + 1, so
+ *    its pointer walks (pointr, ipntr, jpntr) become plain column loops;
+ *  - the 2col-vectors of bmv, cauchy, cmprlb and subsm are held as their two
+ *    col-blocks (elements 1..col and col+1..2col), except c, which cauchy
+ *    leaves for cmprlb and which keeps the reference's flat layout;
+ *  - the 2col x 2col middle matrix K (wn) is held as its blocks K11, K12,
+ *    K22 (wn11, wn12, wn22);
+ *  - loops bounded by col, nsub, nenter, ileave or nbreak run to their
+ *    fixed maximum under a guard;
+ *  - data-dependent subscripts into 2-vectors (ind[k], ibp, ...) are
+ *    two-way selects.
+ * Values, operations and their order are those of the reference routines,
+ * so the iterates are bit-identical to the reference's. */
+#ifdef __HIPCC__
+#define LB_UNROLL _Pragma("unroll")
+#else
+#define LB_UNROLL
+#endif
+
+/* Element i (1-based, i in {1, 2}) of a 2-vector.  Both elements are read
+ * (written) every time and the value is selected, so that no address
+ * depends on data: the optimiser then keeps such arrays in registers. */
+#define LB_AT(a, i)                                                   \
+    ({                                                                \
+        __typeof__((a)[0]) a0_ = (a)[0], a1_ = (a)[1];                \
+        (i) == 1 ? a0_ : a1_;                                         \
+    })
+#define LB_PUT(a, i, v)                                               \
+    do {                                                              \
+        __typeof__((a)[0]) v_ = (v), a0_ = (a)[0], a1_ = (a)[1];      \
+        const int one_ = (i) == 1;                                    \
+        (a)[0] = one_ ? v_ : a0_;                                     \
+        (a)[1] = one_ ? a1_ : v_;                                     \
+    } while (0)
+
 typedef struct fsx_lb {
-    double ws[FSX_LB_N * FSX_LB_M], wy[FSX_LB_N * FSX_LB_M];
-    double sy[FSX_LB_M * FSX_LB_M], ss[FSX_LB_M * FSX_LB_M], wt[FSX_LB_M * FSX_LB_M];
-    double wn[4 * FSX_LB_M * FSX_LB_M], snd[4 * FSX_LB_M * FSX_LB_M];
-    double z[FSX_LB_N], r[FSX_LB_N], d[FSX_LB_N], t[FSX_LB_N], xp[FSX_LB_N];
-    double wa[8 * FSX_LB_M];
+    double ws[FSX_LB_N][FSX_LB_M], wy[FSX_LB_N][FSX_LB_M];   /* [variable][pair] */
+    double sy[FSX_LB_M][FSX_LB_M], ss[FSX_LB_M][FSX_LB_M], wt[FSX_LB_M][FSX_LB_M];
+    double wn1[2 * FSX_LB_M][2 * FSX_LB_M];
+    double wn11[FSX_LB_M][FSX_LB_M], wn12[FSX_LB_M][FSX_LB_M], wn22[FSX_LB_M][FSX_LB_M];
+    double wc[2 * FSX_LB_M];     /* c of cauchy, flat: c(1..col), c(col+1..2col) */
+    double z[FSX_LB_N], r[FSX_LB_N], d[FSX_LB_N], t[FSX_LB_N];
     int index[FSX_LB_N], iwhere[FSX_LB_N], indx2[FSX_LB_N];
 } fsx_lb;
 
-#define LB_WS(i, j) w->ws[((i) - 1) + ((j) - 1) * FSX_LB_N]
-#define LB_WY(i, j) w->wy[((i) - 1) + ((j) - 1) * FSX_LB_N]
-#define LB_SY(i, j) w->sy[((i) - 1) + ((j) - 1) * FSX_LB_M]
-#define LB_SS(i, j) w->ss[((i) - 1) + ((j) - 1) * FSX_LB_M]
-#define LB_WT(i, j) w->wt[((i) - 1) + ((j) - 1) * FSX_LB_M]
-#define LB_WN(i, j) w->wn[((i) - 1) + ((j) - 1) * 2 * FSX_LB_M]
-#define LB_WN1(i, j) w->snd[((i) - 1) + ((j) - 1) * 2 * FSX_LB_M]
+/* 1-based accessors; LB_WSK / LB_WYK take a variable number k in {1, 2}
+ * that may be data dependent */
+#define LB_WS(i, j) w->ws[(i) - 1][(j) - 1]
+#define LB_WY(i, j) w->wy[(i) - 1][(j) - 1]
+#define LB_WSK(k, j)                                                  \
+    ({                                                                \
+        double s0_ = w->ws[0][(j) - 1], s1_ = w->ws[1][(j) - 1];      \
+        (k) == 1 ? s0_ : s1_;                                         \
+    })
+#define LB_WYK(k, j)                                                  \
+    ({                                                                \
+        double y0_ = w->wy[0][(j) - 1], y1_ = w->wy[1][(j) - 1];      \
+        (k) == 1 ? y0_ : y1_;                                         \
+    })
+#define LB_SY(i, j) w->sy[(i) - 1][(j) - 1]
+#define LB_SS(i, j) w->ss[(i) - 1][(j) - 1]
+#define LB_WT(i, j) w->wt[(i) - 1][(j) - 1]
+#define LB_WN1(i, j) w->wn1[(i) - 1][(j) - 1]
 
 FSX_HD double lb_max(double a, double b) { return a >= b ? a : b; }
 FSX_HD double lb_min(double a, double b) { return a <= b ? a : b; }
 FSX_HD double lb_abs(double a) { return a >= 0 ? a : -a; }
 
-/* miniCBLAS.c: ddot is a left-to-right sum from 0 */
-FSX_HD double lb_dot(int n, const double *x, const double *y)
+/* ddot (miniCBLAS.c) is a left-to-right sum from 0.0; daxpy returns at once
+ * when the factor is 0.  Both are written out in place below, apart from
+ * the n-vector dot product. */
+FSX_HD double lb_dot2(const double x[FSX_LB_N], const double y[FSX_LB_N])
 {
     double s = 0.;
-    for (int i = 0; i < n; i++) s = s + x[i] * y[i];
+    s = s + x[0] * y[0];
+    s = s + x[1] * y[1];
     return s;
 }
-FSX_HD void lb_axpy(int n, double a, const double *x, double *y)
-{
-    if (n <= 0 || a == 0.) return;
-    for (int i = 0; i < n; i++) y[i] += a * x[i];
-}
 
-/* linpack.c dpofa on a(lda, n) */
-FSX_HD int lb_chol(double *a, int lda, int n)
+/* linpack.c dpofa: upper Cholesky factor of the leading n x n block of a
+ * (a[i-1][j-1] = A(i, j)), n <= m.  0, or the column that failed. */
+FSX_HD int lb_chol(double a[FSX_LB_M][FSX_LB_M], int n)
 {
-    for (int j = 1; j <= n; ++j) {
-        double s = 0.;
-        for (int k = 1; k <= j - 1; ++k) {
-            double t = a[(k - 1) + (j - 1) * lda]
-                     - lb_dot(k - 1, &a[(k - 1) * lda], &a[(j - 1) * lda]);
-            t /= a[(k - 1) + (k - 1) * lda];
-            a[(k - 1) + (j - 1) * lda] = t;
-            s += t * t;
+    LB_UNROLL
+    for (int j = 1; j <= FSX_LB_M; ++j) {
+        if (j <= n) {
+            double s = 0.;
+            LB_UNROLL
+            for (int k = 1; k <= j - 1; ++k) {
+                double dt = 0.;
+                LB_UNROLL
+                for (int i = 1; i <= k - 1; ++i) dt = dt + a[i - 1][k - 1] * a[i - 1][j - 1];
+                double t = a[k - 1][j - 1] - dt;
+                t /= a[k - 1][k - 1];
+                a[k - 1][j - 1] = t;
+                s += t * t;
+            }
+            s = a[j - 1][j - 1] - s;
+            if (s <= 0.) return j;
+            a[j - 1][j - 1] = sqrt(s);
         }
-        s = a[(j - 1) + (j - 1) * lda] - s;
-        if (s <= 0.) return j;
-        a[(j - 1) + (j - 1) * lda] = sqrt(s);
     }
     return 0;
 }
 
-/* linpack.c dtrsl, job 01 (T x = b) and job 11 (T' x = b), T upper */
-FSX_HD int lb_trsl(const double *t, int ldt, int n, double *b, int job)
+/* linpack.c dtrsl with T the leading n x n upper triangle of t: job 11
+ * solves T' x = b, job 01 solves T x = b, in place.  0, or the first zero
+ * diagonal. */
+FSX_HD int lb_trsl(double t[FSX_LB_M][FSX_LB_M], int n, double b[FSX_LB_M], int job)
 {
-    for (int i = 1; i <= n; ++i)
-        if (t[(i - 1) + (i - 1) * ldt] == 0.) return i;
+    LB_UNROLL
+    for (int i = 1; i <= FSX_LB_M; ++i)
+        if (i <= n && t[i - 1][i - 1] == 0.) return i;
     if (job == 1) {
-        b[n - 1] /= t[(n - 1) + (n - 1) * ldt];
-        for (int jj = 2; jj <= n; ++jj) {
-            int j = n - jj + 1;
-            double tmp = -b[j];
-            lb_axpy(j, tmp, &t[j * ldt], b);
-            b[j - 1] /= t[(j - 1) + (j - 1) * ldt];
+        LB_UNROLL
+        for (int c = 1; c <= FSX_LB_M; ++c)
+            if (c == n) b[c - 1] /= t[c - 1][c - 1];
+        /* columns n .. 2: b(1..j) += -b(j+1) T(1..j, j+1), b(j) /= T(j, j) */
+        LB_UNROLL
+        for (int j = FSX_LB_M - 1; j >= 1; --j) {
+            if (j <= n - 1) {
+                double tmp = -b[j];
+                if (tmp != 0.) {
+                    LB_UNROLL
+                    for (int i = 0; i < j; i++) b[i] += tmp * t[i][j];
+                }
+                b[j - 1] /= t[j - 1][j - 1];
+            }
         }
     } else {
-        b[0] /= t[0];
-        for (int j = 2; j <= n; ++j) {
-            b[j - 1] -= lb_dot(j - 1, &t[(j - 1) * ldt], b);
-            b[j - 1] /= t[(j - 1) + (j - 1) * ldt];
+        b[0] /= t[0][0];
+        LB_UNROLL
+        for (int j = 2; j <= FSX_LB_M; ++j) {
+            if (j <= n) {
+                double dt = 0.;
+                LB_UNROLL
+                for (int i = 0; i < j - 1; i++) dt = dt + t[i][j - 1] * b[i];
+                b[j - 1] -= dt;
+                b[j - 1] /= t[j - 1][j - 1];
+            }
         }
     }
     return 0;
 }
 
-/* subalgorithms.c bmv */
-FSX_HD int lb_bmv(fsx_lb *w, int col, const double *v, double *p)
+/* dtrsl on the 2col x 2col upper triangle K = [K11 K12; 0 K22] held as
+ * its blocks, b = (b1, b2): job 11 solves K' x = b, job 01 solves K x = b.
+ * The element sequence of every sum and update is that of dtrsl on the
+ * assembled matrix. */
+FSX_HD int lb_trsl_k(const fsx_lb *w, int col, double b1[FSX_LB_M], double b2[FSX_LB_M],
+                     int job)
+{
+    const int m = FSX_LB_M;
+    LB_UNROLL
+    for (int i = 1; i <= m; ++i)
+        if (i <= col && w->wn11[i - 1][i - 1] == 0.) return i;
+    LB_UNROLL
+    for (int i = 1; i <= m; ++i)
+        if (i <= col && w->wn22[i - 1][i - 1] == 0.) return col + i;
+    if (job == 1) {
+        LB_UNROLL
+        for (int c = 1; c <= m; ++c)
+            if (c == col) b2[c - 1] /= w->wn22[c - 1][c - 1];
+        /* columns 2col .. col+1 */
+        LB_UNROLL
+        for (int cp = m; cp >= 1; --cp) {
+            if (cp <= col) {
+                double tmp = -b2[cp - 1];
+                if (tmp != 0.) {
+                    LB_UNROLL
+                    for (int i = 0; i < m; i++)
+                        if (i < col) b1[i] += tmp * w->wn12[i][cp - 1];
+                    LB_UNROLL
+                    for (int i = 0; i < cp - 1; i++) b2[i] += tmp * w->wn22[i][cp - 1];
+                }
+                if (cp >= 2) {
+                    b2[cp - 2] /= w->wn22[cp - 2][cp - 2];
+                } else {
+                    LB_UNROLL
+                    for (int c = 1; c <= m; ++c)
+                        if (c == col) b1[c - 1] /= w->wn11[c - 1][c - 1];
+                }
+            }
+        }
+        /* columns col .. 2 */
+        LB_UNROLL
+        for (int c = m; c >= 2; --c) {
+            if (c <= col) {
+                double tmp = -b1[c - 1];
+                if (tmp != 0.) {
+                    LB_UNROLL
+                    for (int i = 0; i < c - 1; i++) b1[i] += tmp * w->wn11[i][c - 1];
+                }
+                b1[c - 2] /= w->wn11[c - 2][c - 2];
+            }
+        }
+    } else {
+        b1[0] /= w->wn11[0][0];
+        LB_UNROLL
+        for (int j = 2; j <= m; ++j) {
+            if (j <= col) {
+                double dt = 0.;
+                LB_UNROLL
+                for (int i = 0; i < j - 1; i++) dt = dt + w->wn11[i][j - 1] * b1[i];
+                b1[j - 1] -= dt;
+                b1[j - 1] /= w->wn11[j - 1][j - 1];
+            }
+        }
+        LB_UNROLL
+        for (int jp = 1; jp <= m; ++jp) {
+            if (jp <= col) {
+                double dt = 0.;
+                LB_UNROLL
+                for (int i = 0; i < m; i++)
+                    if (i < col) dt = dt + w->wn12[i][jp - 1] * b1[i];
+                LB_UNROLL
+                for (int i = 0; i < jp - 1; i++) dt = dt + w->wn22[i][jp - 1] * b2[i];
+                b2[jp - 1] -= dt;
+                b2[jp - 1] /= w->wn22[jp - 1][jp - 1];
+            }
+        }
+    }
+    return 0;
+}
+
+/* subalgorithms.c bmv: (p1, p2) = M (v1, v2), M the 2col x 2col middle
+ * matrix of the compact L-BFGS representation */
+FSX_HD int lb_bmv(fsx_lb *w, int col, const double v1[FSX_LB_M], const double v2[FSX_LB_M],
+                  double p1[FSX_LB_M], double p2[FSX_LB_M])
 {
     if (col == 0) return 0;
-    p[col] = v[col];
-    for (int i = 2; i <= col; ++i) {
-        int i2 = col + i;
-        double sum = 0.;
-        for (int k = 1; k <= i - 1; ++k) sum += LB_SY(i, k) * v[k - 1] / LB_SY(k, k);
-        p[i2 - 1] = v[i2 - 1] + sum;
+    p2[0] = v2[0];
+    LB_UNROLL
+    for (int i = 2; i <= FSX_LB_M; ++i) {
+        if (i <= col) {
+            double sum = 0.;
+            LB_UNROLL
+            for (int k = 1; k <= i - 1; ++k) sum += LB_SY(i, k) * v1[k - 1] / LB_SY(k, k);
+            p2[i - 1] = v2[i - 1] + sum;
+        }
     }
-    int info = lb_trsl(w->wt, FSX_LB_M, col, &p[col], 11);
+    int info = lb_trsl(w->wt, col, p2, 11);
     if (info) return info;
-    for (int i = 1; i <= col; ++i) p[i - 1] = v[i - 1] / sqrt(LB_SY(i, i));
-    info = lb_trsl(w->wt, FSX_LB_M, col, &p[col], 1);
+    LB_UNROLL
+    for (int i = 1; i <= FSX_LB_M; ++i)
+        if (i <= col) p1[i - 1] = v1[i - 1] / sqrt(LB_SY(i, i));
+    info = lb_trsl(w->wt, col, p2, 1);
     if (info) return info;
-    for (int i = 1; i <= col; ++i) p[i - 1] = -p[i - 1] / sqrt(LB_SY(i, i));
-    for (int i = 1; i <= col; ++i) {
-        double sum = 0.;
-        for (int k = i + 1; k <= col; ++k) sum += LB_SY(k, i) * p[col + k - 1] / LB_SY(i, i);
-        p[i - 1] += sum;
+    LB_UNROLL
+    for (int i = 1; i <= FSX_LB_M; ++i)
+        if (i <= col) p1[i - 1] = -p1[i - 1] / sqrt(LB_SY(i, i));
+    LB_UNROLL
+    for (int i = 1; i <= FSX_LB_M; ++i) {
+        if (i <= col) {
+            double sum = 0.;
+            LB_UNROLL
+            for (int k = i + 1; k <= FSX_LB_M; ++k)
+                if (k <= col) sum += LB_SY(k, i) * p2[k - 1] / LB_SY(i, i);
+            p1[i - 1] += sum;
+        }
     }
     return 0;
 }
 
-/* subalgorithms.c hpsolb on 1-based views */
-FSX_HD void lb_hpsolb(int n, double *t, int *iorder, int iheap)
+/* element j (0-based, < col) of the second block of the flat c */
+FSX_HD double lb_c2(const fsx_lb *w, int col, int j)
 {
-    if (iheap == 0) {
-        for (int k = 2; k <= n; ++k) {
-            double dd = t[(k) - 1];
-            int in = iorder[(k) - 1];
-            int i = k;
-            while (i > 1) {
-                int j = i / 2;
-                if (dd < t[(j) - 1]) { t[(i) - 1] = t[(j) - 1]; iorder[(i) - 1] = iorder[(j) - 1]; i = j; }
-                else break;
-            }
-            t[(i) - 1] = dd;
-            iorder[(i) - 1] = in;
-        }
-    }
-    if (n > 1) {
-        int i = 1;
-        double out = t[(1) - 1];
-        int iou = iorder[(1) - 1];
-        double dd = t[(n) - 1];
-        int in = iorder[(n) - 1];
-        for (;;) {
-            int j = i + i;
-            if (j <= n - 1) {
-                if (t[(j + 1) - 1] < t[(j) - 1]) ++j;
-                if (t[(j) - 1] < dd) { t[(i) - 1] = t[(j) - 1]; iorder[(i) - 1] = iorder[(j) - 1]; i = j; continue; }
-            }
-            break;
-        }
-        t[(i) - 1] = dd;
-        iorder[(i) - 1] = in;
-        t[(n) - 1] = out;
-        iorder[(n) - 1] = iou;
-    }
+    double e1 = w->wc[1 + j], e2 = w->wc[2 + j], e3 = w->wc[3 + j];
+    return col == 1 ? e1 : col == 2 ? e2 : e3;
+}
+/* c2[j] += a * x */
+FSX_HD void lb_c2_axpy(fsx_lb *w, int col, int j, double a, double x)
+{
+    double e1 = w->wc[1 + j], e2 = w->wc[2 + j], e3 = w->wc[3 + j];
+    w->wc[1 + j] = col == 1 ? e1 + a * x : e1;
+    w->wc[2 + j] = col == 2 ? e2 + a * x : e2;
+    w->wc[3 + j] = col == 3 ? e3 + a * x : e3;
 }
 
-/* subalgorithms.c cauchy (all bounds of type 2 here: nbd = 2) */
-FSX_HD int lb_cauchy(fsx_lb *w, const double *x0, const double *l0, const double *u0,
-                     const double *g0, double theta, int col, int head, double sbgnrm,
-                     int *nseg)
+/* subalgorithms.c cauchy, every variable bounded on both sides (nbd = 2):
+ * the generalized Cauchy point into w->z and c = W'(xcp - x) into w->wc.
+ * With n = 2 there are at most two breakpoints; the heap of the second one
+ * (hpsolb on one element) leaves it in place. */
+FSX_HD int lb_cauchy(fsx_lb *w, const double x[FSX_LB_N], const double l[FSX_LB_N],
+                     const double u[FSX_LB_N], const double g[FSX_LB_N], double theta,
+                     int col, double sbgnrm, int *nseg)
 {
     const int n = FSX_LB_N, m = FSX_LB_M;
-    const double *x = x0, *l = l0, *u = u0, *g = g0;
     int *iorder = w->indx2, *iwhere = w->iwhere;
     double *t = w->t, *d = w->d, *xcp = w->z;
-    double *p = w->wa, *c = w->wa + 2 * m, *wbp = w->wa + 4 * m, *v = w->wa + 6 * m;
+    double p1[FSX_LB_M], p2[FSX_LB_M], v1[FSX_LB_M], v2[FSX_LB_M];
+    double wbp1[FSX_LB_M], wbp2[FSX_LB_M];
     const double epsmch = DBL_EPSILON;
     if (sbgnrm <= 0.) {
-        for (int i = 0; i < n; i++) w->z[i] = x0[i];
+        xcp[0] = x[0];
+        xcp[1] = x[1];
         return 0;
     }
     int bnded = 1, nfree = n + 1, nbreak = 0, ibkmin = 0;
     double bkmin = 0., f1 = 0., tl = 0., tu = 0.;
-    int col2 = col << 1;
-    for (int i = 0; i < col2; i++) p[i] = 0.;
+    LB_UNROLL
+    for (int j = 0; j < m; j++) {
+        p1[j] = 0.; p2[j] = 0.; v1[j] = 0.; v2[j] = 0.; wbp1[j] = 0.; wbp2[j] = 0.;
+    }
+    LB_UNROLL
     for (int i = 1; i <= n; ++i) {
-        double neggi = -g[(i) - 1];
-        if (iwhere[(i) - 1] != 3 && iwhere[(i) - 1] != -1) {
-            tl = x[(i) - 1] - l[(i) - 1];
-            tu = u[(i) - 1] - x[(i) - 1];
+        double neggi = -g[i - 1];
+        if (iwhere[i - 1] != 3 && iwhere[i - 1] != -1) {
+            tl = x[i - 1] - l[i - 1];
+            tu = u[i - 1] - x[i - 1];
             int xlower = tl <= 0.;
             int xupper = tu <= 0.;
-            iwhere[(i) - 1] = 0;
+            iwhere[i - 1] = 0;
             if (xlower) {
-                if (neggi <= 0.) iwhere[(i) - 1] = 1;
+                if (neggi <= 0.) iwhere[i - 1] = 1;
             } else if (xupper) {
-                if (neggi >= 0.) iwhere[(i) - 1] = 2;
+                if (neggi >= 0.) iwhere[i - 1] = 2;
             } else {
-                if (lb_abs(neggi) <= 0.) iwhere[(i) - 1] = -3;
+                if (lb_abs(neggi) <= 0.) iwhere[i - 1] = -3;
             }
         }
-        int pointr = head;
-        if (iwhere[(i) - 1] != 0 && iwhere[(i) - 1] != -1) {
-            d[(i) - 1] = 0.;
+        if (iwhere[i - 1] != 0 && iwhere[i - 1] != -1) {
+            d[i - 1] = 0.;
         } else {
-            d[(i) - 1] = neggi;
+            d[i - 1] = neggi;
             f1 -= neggi * neggi;
-            for (int j = 1; j <= col; ++j) {
-                p[j - 1] += LB_WY(i, pointr) * neggi;
-                p[col + j - 1] += LB_WS(i, pointr) * neggi;
-                pointr = pointr % m + 1;
+            LB_UNROLL
+            for (int j = 1; j <= m; ++j) {
+                if (j <= col) {
+                    p1[j - 1] += LB_WY(i, j) * neggi;
+                    p2[j - 1] += LB_WS(i, j) * neggi;
+                }
             }
             if (neggi < 0.) {
                 ++nbreak;
-                iorder[(nbreak) - 1] = i;
-                t[(nbreak) - 1] = tl / (-neggi);
-                if (nbreak == 1 || t[(nbreak) - 1] < bkmin) { bkmin = t[(nbreak) - 1]; ibkmin = nbreak; }
+                LB_PUT(iorder, nbreak, i);
+                LB_PUT(t, nbreak, tl / (-neggi));
+                if (nbreak == 1 || LB_AT(t, nbreak) < bkmin) {
+                    bkmin = LB_AT(t, nbreak);
+                    ibkmin = nbreak;
+                }
             } else if (neggi > 0.) {
                 ++nbreak;
-                iorder[(nbreak) - 1] = i;
-                t[(nbreak) - 1] = tu / neggi;
-                if (nbreak == 1 || t[(nbreak) - 1] < bkmin) { bkmin = t[(nbreak) - 1]; ibkmin = nbreak; }
+                LB_PUT(iorder, nbreak, i);
+                LB_PUT(t, nbreak, tu / neggi);
+                if (nbreak == 1 || LB_AT(t, nbreak) < bkmin) {
+                    bkmin = LB_AT(t, nbreak);
+                    ibkmin = nbreak;
+                }
             } else {
                 --nfree;
-                iorder[(nfree) - 1] = i;
+                LB_PUT(iorder, nfree, i);
                 if (lb_abs(neggi) > 0.) bnded = 0;
             }
         }
     }
-    if (theta != 1.)
-        for (int i = 0; i < col; i++) p[col + i] = theta * p[col + i];
-    for (int i = 0; i < n; i++) w->z[i] = x0[i];
+    if (theta != 1.) {
+        LB_UNROLL
+        for (int j = 0; j < m; j++)
+            if (j < col) p2[j] = theta * p2[j];
+    }
+    xcp[0] = x[0];
+    xcp[1] = x[1];
     if (nbreak == 0 && nfree == n + 1) return 0;
-    for (int j = 0; j < col2; j++) c[j] = 0.;
+    LB_UNROLL
+    for (int j = 0; j < 2 * m; j++)
+        if (j < 2 * col) w->wc[j] = 0.;
     double f2 = -theta * f1;
     double f2_org = f2;
     if (col > 0) {
-        int info = lb_bmv(w, col, p, v);
+        int info = lb_bmv(w, col, p1, p2, v1, v2);
         if (info) return info;
-        f2 -= lb_dot(col2, v, p);
+        double dt = 0.;
+        LB_UNROLL
+        for (int j = 0; j < m; j++)
+            if (j < col) dt = dt + v1[j] * p1[j];
+        LB_UNROLL
+        for (int j = 0; j < m; j++)
+            if (j < col) dt = dt + v2[j] * p2[j];
+        f2 -= dt;
     }
     double dtm = -f1 / f2;
     double tsum = 0.;
     *nseg = 1;
     int reached_end = 0;
     if (nbreak != 0) {
-        int nleft = nbreak, iter = 1, ibp;
+        int nleft = nbreak, ibp = 0;
         double tj = 0.;
-        for (;;) {
+        LB_UNROLL
+        for (int iter = 1; iter <= n; ++iter) {
             double tj0 = tj;
             if (iter == 1) {
                 tj = bkmin;
-                ibp = iorder[(ibkmin) - 1];
+                ibp = LB_AT(iorder, ibkmin);
             } else {
-                if (iter == 2 && ibkmin != nbreak) {
-                    t[(ibkmin) - 1] = t[(nbreak) - 1];
-                    iorder[(ibkmin) - 1] = iorder[(nbreak) - 1];
+                if (ibkmin != nbreak) {
+                    LB_PUT(t, ibkmin, LB_AT(t, nbreak));
+                    LB_PUT(iorder, ibkmin, LB_AT(iorder, nbreak));
                 }
-                lb_hpsolb(nleft, t, iorder, iter - 2);
-                tj = t[(nleft) - 1];
-                ibp = iorder[(nleft) - 1];
+                tj = LB_AT(t, nleft);
+                ibp = LB_AT(iorder, nleft);
             }
             double dt = tj - tj0;
             if (dtm < dt) break;
             tsum += dt;
             --nleft;
-            ++iter;
-            double dibp = d[(ibp) - 1];
-            d[(ibp) - 1] = 0.;
+            double dibp = LB_AT(d, ibp);
+            LB_PUT(d, ibp, 0.);
             double zibp;
             if (dibp > 0.) {
-                zibp = u[(ibp) - 1] - x[(ibp) - 1];
-                xcp[(ibp) - 1] = u[(ibp) - 1];
-                iwhere[(ibp) - 1] = 2;
+                zibp = LB_AT(u, ibp) - LB_AT(x, ibp);
+                LB_PUT(xcp, ibp, LB_AT(u, ibp));
+                LB_PUT(iwhere, ibp, 2);
             } else {
-                zibp = l[(ibp) - 1] - x[(ibp) - 1];
-                xcp[(ibp) - 1] = l[(ibp) - 1];
-                iwhere[(ibp) - 1] = 1;
+                zibp = LB_AT(l, ibp) - LB_AT(x, ibp);
+                LB_PUT(xcp, ibp, LB_AT(l, ibp));
+                LB_PUT(iwhere, ibp, 1);
             }
             if (nleft == 0 && nbreak == n) {
                 dtm = dt;
@@ -526,19 +699,48 @@ FSX_HD int lb_cauchy(fsx_lb *w, const double *x0, const double *l0, const double
             f1 = f1 + dt * f2 + dibp2 - theta * dibp * zibp;
             f2 -= theta * dibp2;
             if (col > 0) {
-                lb_axpy(col2, dt, p, c);
-                int pointr = head;
-                for (int j = 1; j <= col; ++j) {
-                    wbp[j - 1] = LB_WY(ibp, pointr);
-                    wbp[col + j - 1] = theta * LB_WS(ibp, pointr);
-                    pointr = pointr % m + 1;
+                if (dt != 0.) {
+                    LB_UNROLL
+                    for (int j = 0; j < m; j++)
+                        if (j < col) w->wc[j] += dt * p1[j];
+                    LB_UNROLL
+                    for (int j = 0; j < m; j++)
+                        if (j < col) lb_c2_axpy(w, col, j, dt, p2[j]);
                 }
-                int info = lb_bmv(w, col, wbp, v);
+                LB_UNROLL
+                for (int j = 1; j <= m; ++j) {
+                    if (j <= col) {
+                        wbp1[j - 1] = LB_WYK(ibp, j);
+                        wbp2[j - 1] = theta * LB_WSK(ibp, j);
+                    }
+                }
+                int info = lb_bmv(w, col, wbp1, wbp2, v1, v2);
                 if (info) return info;
-                double wmc = lb_dot(col2, c, v);
-                double wmp = lb_dot(col2, p, v);
-                double wmw = lb_dot(col2, wbp, v);
-                lb_axpy(col2, -dibp, wbp, p);
+                double wmc = 0., wmp = 0., wmw = 0.;
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmc = wmc + w->wc[j] * v1[j];
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmc = wmc + lb_c2(w, col, j) * v2[j];
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmp = wmp + p1[j] * v1[j];
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmp = wmp + p2[j] * v2[j];
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmw = wmw + wbp1[j] * v1[j];
+                LB_UNROLL
+                for (int j = 0; j < m; j++)
+                    if (j < col) wmw = wmw + wbp2[j] * v2[j];
+                const double a = -dibp;
+                if (a != 0.) {
+                    LB_UNROLL
+                    for (int j = 0; j < m; j++)
+                        if (j < col) { p1[j] += a * wbp1[j]; p2[j] += a * wbp2[j]; }
+                }
                 f1 += dibp * wmc;
                 f2 = f2 + dibp * 2. * wmp - dibp2 * wmw;
             }
@@ -547,7 +749,9 @@ FSX_HD int lb_cauchy(fsx_lb *w, const double *x0, const double *l0, const double
                 dtm = -f1 / f2;
                 continue;
             } else if (bnded) {
-                f1 = 0.; f2 = 0.; dtm = 0.;
+                f1 = 0.;
+                f2 = 0.;
+                dtm = 0.;
             } else {
                 dtm = -f1 / f2;
             }
@@ -557,9 +761,19 @@ FSX_HD int lb_cauchy(fsx_lb *w, const double *x0, const double *l0, const double
     if (!reached_end) {
         if (dtm <= 0.) dtm = 0.;
         tsum += dtm;
-        lb_axpy(n, tsum, w->d, w->z);
+        if (tsum != 0.) {
+            xcp[0] += tsum * d[0];
+            xcp[1] += tsum * d[1];
+        }
     }
-    if (col > 0) lb_axpy(col2, dtm, p, c);
+    if (col > 0 && dtm != 0.) {
+        LB_UNROLL
+        for (int j = 0; j < m; j++)
+            if (j < col) w->wc[j] += dtm * p1[j];
+        LB_UNROLL
+        for (int j = 0; j < m; j++)
+            if (j < col) lb_c2_axpy(w, col, j, dtm, p2[j]);
+    }
     return 0;
 }
 
@@ -572,384 +786,550 @@ FSX_HD void lb_freev(fsx_lb *w, int *nfree, int *nenter, int *ileave, int *wrk,
     *nenter = 0;
     *ileave = n + 1;
     if (iter > 0) {
-        for (int i = 1; i <= *nfree; ++i) {
-            int k = index[(i) - 1];
-            if (iwhere[(k) - 1] > 0) { --(*ileave); indx2[(*ileave) - 1] = k; }
+        LB_UNROLL
+        for (int i = 1; i <= n; ++i) {
+            if (i <= *nfree) {
+                int k = index[i - 1];
+                if (LB_AT(iwhere, k) > 0) {
+                    --(*ileave);
+                    LB_PUT(indx2, *ileave, k);
+                }
+            }
         }
-        for (int i = *nfree + 1; i <= n; ++i) {
-            int k = index[(i) - 1];
-            if (iwhere[(k) - 1] <= 0) { ++(*nenter); indx2[(*nenter) - 1] = k; }
+        LB_UNROLL
+        for (int i = 1; i <= n; ++i) {
+            if (i >= *nfree + 1) {
+                int k = index[i - 1];
+                if (LB_AT(iwhere, k) <= 0) {
+                    ++(*nenter);
+                    LB_PUT(indx2, *nenter, k);
+                }
+            }
         }
     }
     *wrk = *ileave < n + 1 || *nenter > 0 || updatd;
     *nfree = 0;
     int iact = n + 1;
+    LB_UNROLL
     for (int i = 1; i <= n; ++i) {
-        if (iwhere[(i) - 1] <= 0) { ++(*nfree); index[(*nfree) - 1] = i; }
-        else { --iact; index[(iact) - 1] = i; }
+        if (iwhere[i - 1] <= 0) {
+            ++(*nfree);
+            LB_PUT(index, *nfree, i);
+        } else {
+            --iact;
+            LB_PUT(index, iact, i);
+        }
     }
 }
 
-/* subalgorithms.c formk */
-FSX_HD int lb_formk(fsx_lb *w, int nsub, int nenter, int ileave, int iupdat,
-                    int updatd, double theta, int col, int head)
+/* subalgorithms.c formk: update WN1 (the inner products of the pairs over
+ * the free and the active variables) and factor K = [K11 K12; 0 K22]. */
+FSX_HD int lb_formk(fsx_lb *w, int nsub, int nenter, int ileave, int iupdat, int updatd,
+                    double theta, int col)
 {
     const int n = FSX_LB_N, m = FSX_LB_M;
     const int *ind = w->index, *indx2 = w->indx2;
     int upcl;
     if (updatd) {
         if (iupdat > m) {
+            LB_UNROLL
             for (int jy = 1; jy <= m - 1; ++jy) {
-                int js = m + jy;
+                const int js = m + jy;
+                LB_UNROLL
                 for (int k = 0; k < m - jy; k++) LB_WN1(jy + k, jy) = LB_WN1(jy + 1 + k, jy + 1);
+                LB_UNROLL
                 for (int k = 0; k < m - jy; k++) LB_WN1(js + k, js) = LB_WN1(js + 1 + k, js + 1);
+                LB_UNROLL
                 for (int k = 0; k < m - 1; k++) LB_WN1(m + 1 + k, jy) = LB_WN1(m + 2 + k, jy + 1);
             }
         }
-        int iy = col, is = m + col;
-        int ipntr = head + col - 1;
-        if (ipntr > m) ipntr -= m;
-        int jpntr = head;
-        for (int jy = 1; jy <= col; ++jy) {
-            int js = m + jy;
-            double t1 = 0., t2 = 0., t3 = 0.;
-            for (int k = 1; k <= nsub; ++k) {
-                int k1 = ind[(k) - 1];
-                t1 += LB_WY(k1, ipntr) * LB_WY(k1, jpntr);
+        /* the row and column of the newest pair (column col) */
+        LB_UNROLL
+        for (int cc = 1; cc <= m; ++cc) {
+            if (cc == col) {
+                const int iy = cc, is = m + cc;
+                LB_UNROLL
+                for (int jy = 1; jy <= cc; ++jy) {
+                    const int js = m + jy;
+                    double t1 = 0., t2 = 0., t3 = 0.;
+                    LB_UNROLL
+                    for (int k = 1; k <= n; ++k) {
+                        if (k <= nsub) {
+                            int k1 = ind[k - 1];
+                            t1 += LB_WYK(k1, cc) * LB_WYK(k1, jy);
+                        }
+                    }
+                    LB_UNROLL
+                    for (int k = 1; k <= n; ++k) {
+                        if (k >= nsub + 1) {
+                            int k1 = ind[k - 1];
+                            t2 += LB_WSK(k1, cc) * LB_WSK(k1, jy);
+                            t3 += LB_WSK(k1, cc) * LB_WYK(k1, jy);
+                        }
+                    }
+                    LB_WN1(iy, jy) = t1;
+                    LB_WN1(is, js) = t2;
+                    LB_WN1(is, jy) = t3;
+                }
+                LB_UNROLL
+                for (int i = 1; i <= cc; ++i) {
+                    const int is2 = m + i;
+                    double t3 = 0.;
+                    LB_UNROLL
+                    for (int k = 1; k <= n; ++k) {
+                        if (k <= nsub) {
+                            int k1 = ind[k - 1];
+                            t3 += LB_WSK(k1, i) * LB_WYK(k1, cc);
+                        }
+                    }
+                    LB_WN1(is2, cc) = t3;
+                }
             }
-            for (int k = nsub + 1; k <= n; ++k) {
-                int k1 = ind[(k) - 1];
-                t2 += LB_WS(k1, ipntr) * LB_WS(k1, jpntr);
-                t3 += LB_WS(k1, ipntr) * LB_WY(k1, jpntr);
-            }
-            LB_WN1(iy, jy) = t1;
-            LB_WN1(is, js) = t2;
-            LB_WN1(is, jy) = t3;
-            jpntr = jpntr % m + 1;
-        }
-        int jy = col;
-        jpntr = head + col - 1;
-        if (jpntr > m) jpntr -= m;
-        ipntr = head;
-        for (int i = 1; i <= col; ++i) {
-            int is2 = m + i;
-            double t3 = 0.;
-            for (int k = 1; k <= nsub; ++k) {
-                int k1 = ind[(k) - 1];
-                t3 += LB_WS(k1, ipntr) * LB_WY(k1, jpntr);
-            }
-            ipntr = ipntr % m + 1;
-            LB_WN1(is2, jy) = t3;
         }
         upcl = col - 1;
     } else {
         upcl = col;
     }
-    int ipntr = head;
-    for (int iy = 1; iy <= upcl; ++iy) {
-        int is = m + iy;
-        int jpntr = head;
-        for (int jy = 1; jy <= iy; ++jy) {
-            int js = m + jy;
-            double t1 = 0., t2 = 0., t3 = 0., t4 = 0.;
-            for (int k = 1; k <= nenter; ++k) {
-                int k1 = indx2[(k) - 1];
-                t1 += LB_WY(k1, ipntr) * LB_WY(k1, jpntr);
-                t2 += LB_WS(k1, ipntr) * LB_WS(k1, jpntr);
+    /* the older pairs: variables that entered or left the free set */
+    LB_UNROLL
+    for (int iy = 1; iy <= m; ++iy) {
+        if (iy <= upcl) {
+            const int is = m + iy;
+            LB_UNROLL
+            for (int jy = 1; jy <= iy; ++jy) {
+                const int js = m + jy;
+                double t1 = 0., t2 = 0., t3 = 0., t4 = 0.;
+                LB_UNROLL
+                for (int k = 1; k <= n; ++k) {
+                    if (k <= nenter) {
+                        int k1 = indx2[k - 1];
+                        t1 += LB_WYK(k1, iy) * LB_WYK(k1, jy);
+                        t2 += LB_WSK(k1, iy) * LB_WSK(k1, jy);
+                    }
+                }
+                LB_UNROLL
+                for (int k = 1; k <= n; ++k) {
+                    if (k >= ileave) {
+                        int k1 = indx2[k - 1];
+                        t3 += LB_WYK(k1, iy) * LB_WYK(k1, jy);
+                        t4 += LB_WSK(k1, iy) * LB_WSK(k1, jy);
+                    }
+                }
+                LB_WN1(iy, jy) = LB_WN1(iy, jy) + t1 - t3;
+                LB_WN1(is, js) = LB_WN1(is, js) - t2 + t4;
             }
-            for (int k = ileave; k <= n; ++k) {
-                int k1 = indx2[(k) - 1];
-                t3 += LB_WY(k1, ipntr) * LB_WY(k1, jpntr);
-                t4 += LB_WS(k1, ipntr) * LB_WS(k1, jpntr);
-            }
-            LB_WN1(iy, jy) = LB_WN1(iy, jy) + t1 - t3;
-            LB_WN1(is, js) = LB_WN1(is, js) - t2 + t4;
-            jpntr = jpntr % m + 1;
         }
-        ipntr = ipntr % m + 1;
     }
-    ipntr = head;
-    for (int is = m + 1; is <= m + upcl; ++is) {
-        int jpntr = head;
-        for (int jy = 1; jy <= upcl; ++jy) {
-            double t1 = 0., t3 = 0.;
-            for (int k = 1; k <= nenter; ++k) {
-                int k1 = indx2[(k) - 1];
-                t1 += LB_WS(k1, ipntr) * LB_WY(k1, jpntr);
+    LB_UNROLL
+    for (int is = m + 1; is <= 2 * m; ++is) {
+        if (is <= m + upcl) {
+            LB_UNROLL
+            for (int jy = 1; jy <= m; ++jy) {
+                if (jy <= upcl) {
+                    double t1 = 0., t3 = 0.;
+                    LB_UNROLL
+                    for (int k = 1; k <= n; ++k) {
+                        if (k <= nenter) {
+                            int k1 = indx2[k - 1];
+                            t1 += LB_WSK(k1, is - m) * LB_WYK(k1, jy);
+                        }
+                    }
+                    LB_UNROLL
+                    for (int k = 1; k <= n; ++k) {
+                        if (k >= ileave) {
+                            int k1 = indx2[k - 1];
+                            t3 += LB_WSK(k1, is - m) * LB_WYK(k1, jy);
+                        }
+                    }
+                    if (is <= jy + m) LB_WN1(is, jy) = LB_WN1(is, jy) + t1 - t3;
+                    else LB_WN1(is, jy) = LB_WN1(is, jy) - t1 + t3;
+                }
             }
-            for (int k = ileave; k <= n; ++k) {
-                int k1 = indx2[(k) - 1];
-                t3 += LB_WS(k1, ipntr) * LB_WY(k1, jpntr);
+        }
+    }
+    /* K11 = Y'Y / theta + D (upper), K22 = theta S'S (upper), K12 = -L_a' + R_z */
+    LB_UNROLL
+    for (int iy = 1; iy <= m; ++iy) {
+        if (iy <= col) {
+            const int is1 = m + iy;
+            LB_UNROLL
+            for (int jy = 1; jy <= iy; ++jy) {
+                const int js1 = m + jy;
+                w->wn11[jy - 1][iy - 1] = LB_WN1(iy, jy) / theta;
+                w->wn22[jy - 1][iy - 1] = LB_WN1(is1, js1) * theta;
             }
-            if (is <= jy + m) LB_WN1(is, jy) = LB_WN1(is, jy) + t1 - t3;
-            else LB_WN1(is, jy) = LB_WN1(is, jy) - t1 + t3;
-            jpntr = jpntr % m + 1;
+            LB_UNROLL
+            for (int jy = 1; jy <= iy - 1; ++jy) w->wn12[jy - 1][iy - 1] = -LB_WN1(is1, jy);
+            LB_UNROLL
+            for (int jy = iy; jy <= m; ++jy)
+                if (jy <= col) w->wn12[jy - 1][iy - 1] = LB_WN1(is1, jy);
+            w->wn11[iy - 1][iy - 1] += LB_SY(iy, iy);
         }
-        ipntr = ipntr % m + 1;
     }
-    const int m2 = m << 1;
-    for (int iy = 1; iy <= col; ++iy) {
-        int is = col + iy, is1 = m + iy;
-        for (int jy = 1; jy <= iy; ++jy) {
-            int js = col + jy, js1 = m + jy;
-            LB_WN(jy, iy) = LB_WN1(iy, jy) / theta;
-            LB_WN(js, is) = LB_WN1(is1, js1) * theta;
+    if (lb_chol(w->wn11, col)) return -1;
+    /* K12 <- L11^{-1} K12, column by column (dtrsl job 11) */
+    LB_UNROLL
+    for (int js = 1; js <= m; ++js) {
+        if (js <= col) {
+            double b[FSX_LB_M];
+            LB_UNROLL
+            for (int i = 0; i < m; i++) b[i] = w->wn12[i][js - 1];
+            lb_trsl(w->wn11, col, b, 11);
+            LB_UNROLL
+            for (int i = 0; i < m; i++) w->wn12[i][js - 1] = b[i];
         }
-        for (int jy = 1; jy <= iy - 1; ++jy) LB_WN(jy, is) = -LB_WN1(is1, jy);
-        for (int jy = iy; jy <= col; ++jy) LB_WN(jy, is) = LB_WN1(is1, jy);
-        LB_WN(iy, iy) += LB_SY(iy, iy);
     }
-    if (lb_chol(w->wn, m2, col)) return -1;
-    int col2 = col << 1;
-    for (int js = col + 1; js <= col2; ++js) lb_trsl(w->wn, m2, col, &LB_WN(1, js), 11);
-    for (int is = col + 1; is <= col2; ++is)
-        for (int js = is; js <= col2; ++js)
-            LB_WN(is, js) += lb_dot(col, &LB_WN(1, is), &LB_WN(1, js));
-    if (lb_chol(&LB_WN(col + 1, col + 1), m2, col)) return -2;
+    LB_UNROLL
+    for (int is = 1; is <= m; ++is) {
+        if (is <= col) {
+            LB_UNROLL
+            for (int js = is; js <= m; ++js) {
+                if (js <= col) {
+                    double dt = 0.;
+                    LB_UNROLL
+                    for (int i = 0; i < m; i++)
+                        if (i < col) dt = dt + w->wn12[i][is - 1] * w->wn12[i][js - 1];
+                    w->wn22[is - 1][js - 1] += dt;
+                }
+            }
+        }
+    }
+    if (lb_chol(w->wn22, col)) return -2;
     return 0;
 }
 
-/* subalgorithms.c formt */
+/* subalgorithms.c formt: T = theta S'S + L D^{-1} L', factored (upper) */
 FSX_HD int lb_formt(fsx_lb *w, int col, double theta)
 {
-    for (int j = 1; j <= col; ++j) LB_WT(1, j) = theta * LB_SS(1, j);
-    for (int i = 2; i <= col; ++i)
-        for (int j = i; j <= col; ++j) {
-            int k1 = (i <= j ? i : j) - 1;
-            double dd = 0.;
-            for (int k = 1; k <= k1; ++k) dd += LB_SY(i, k) * LB_SY(j, k) / LB_SY(k, k);
-            LB_WT(i, j) = dd + theta * LB_SS(i, j);
+    LB_UNROLL
+    for (int j = 1; j <= FSX_LB_M; ++j)
+        if (j <= col) LB_WT(1, j) = theta * LB_SS(1, j);
+    LB_UNROLL
+    for (int i = 2; i <= FSX_LB_M; ++i) {
+        if (i <= col) {
+            LB_UNROLL
+            for (int j = i; j <= FSX_LB_M; ++j) {
+                if (j <= col) {
+                    const int k1 = i - 1;   /* min(i, j) - 1 */
+                    double dd = 0.;
+                    LB_UNROLL
+                    for (int k = 1; k <= k1; ++k) dd += LB_SY(i, k) * LB_SY(j, k) / LB_SY(k, k);
+                    LB_WT(i, j) = dd + theta * LB_SS(i, j);
+                }
+            }
         }
-    return lb_chol(w->wt, FSX_LB_M, col) ? -3 : 0;
+    }
+    return lb_chol(w->wt, col) ? -3 : 0;
 }
 
-/* subalgorithms.c cmprlb (constrained problem) */
-FSX_HD int lb_cmprlb(fsx_lb *w, const double *x0, const double *g0, double theta,
-                     int col, int head, int nfree)
+/* subalgorithms.c cmprlb (constrained problem): reduced gradient of the
+ * quadratic model at the Cauchy point, over the free variables */
+FSX_HD int lb_cmprlb(fsx_lb *w, const double x[FSX_LB_N], const double g[FSX_LB_N],
+                     double theta, int col, int nfree)
 {
     const int m = FSX_LB_M;
-    const double *x = x0, *g = g0, *z = w->z;
+    const double *z = w->z;
     double *r = w->r;
     const int *index = w->index;
-    for (int i = 1; i <= nfree; ++i) {
-        int k = index[(i) - 1];
-        r[(i) - 1] = -theta * (z[(k) - 1] - x[(k) - 1]) - g[(k) - 1];
-    }
-    if (lb_bmv(w, col, w->wa + 2 * m, w->wa)) return -8;
-    int pointr = head;
-    for (int j = 1; j <= col; ++j) {
-        double a1 = w->wa[j - 1], a2 = theta * w->wa[col + j - 1];
-        for (int i = 1; i <= nfree; ++i) {
-            int k = index[(i) - 1];
-            r[(i) - 1] = r[(i) - 1] + LB_WY(k, pointr) * a1 + LB_WS(k, pointr) * a2;
+    LB_UNROLL
+    for (int i = 1; i <= FSX_LB_N; ++i) {
+        if (i <= nfree) {
+            int k = index[i - 1];
+            r[i - 1] = -theta * (LB_AT(z, k) - LB_AT(x, k)) - LB_AT(g, k);
         }
-        pointr = pointr % m + 1;
+    }
+    double c1[FSX_LB_M], c2[FSX_LB_M], o1[FSX_LB_M], o2[FSX_LB_M];
+    LB_UNROLL
+    for (int j = 0; j < m; j++) {
+        c1[j] = w->wc[j];
+        c2[j] = lb_c2(w, col, j);
+        o1[j] = 0.;
+        o2[j] = 0.;
+    }
+    if (lb_bmv(w, col, c1, c2, o1, o2)) return -8;
+    LB_UNROLL
+    for (int j = 1; j <= m; ++j) {
+        if (j <= col) {
+            double a1 = o1[j - 1], a2 = theta * o2[j - 1];
+            LB_UNROLL
+            for (int i = 1; i <= FSX_LB_N; ++i) {
+                if (i <= nfree) {
+                    int k = index[i - 1];
+                    r[i - 1] = r[i - 1] + LB_WYK(k, j) * a1 + LB_WSK(k, j) * a2;
+                }
+            }
+        }
     }
     return 0;
 }
 
-/* subalgorithms.c subsm (3.0, with the projected backtracking safeguard) */
-FSX_HD int lb_subsm(fsx_lb *w, int nsub, const double *l0, const double *u0,
-                    const double *xx0, const double *gg0, double theta, int col,
-                    int head)
+/* subalgorithms.c subsm (3.0, with the projected backtracking safeguard):
+ * subspace minimisation over the free variables, result in w->z */
+FSX_HD int lb_subsm(fsx_lb *w, int nsub, const double l[FSX_LB_N], const double u[FSX_LB_N],
+                    const double xx[FSX_LB_N], const double gg[FSX_LB_N], double theta,
+                    int col)
 {
     const int n = FSX_LB_N, m = FSX_LB_M;
-    const double *l = l0, *u = u0, *xx = xx0, *gg = gg0;
     const int *ind = w->index;
-    double *x = w->z, *d = w->r, *wv = w->wa;
+    double *x = w->z, *d = w->r;
+    double wv1[FSX_LB_M], wv2[FSX_LB_M], xp[FSX_LB_N];
     if (nsub <= 0) return 0;
-    int pointr = head;
-    for (int i = 1; i <= col; ++i) {
-        double t1 = 0., t2 = 0.;
-        for (int j = 1; j <= nsub; ++j) {
-            int k = ind[(j) - 1];
-            t1 += LB_WY(k, pointr) * d[(j) - 1];
-            t2 += LB_WS(k, pointr) * d[(j) - 1];
+    LB_UNROLL
+    for (int i = 1; i <= m; ++i) {
+        wv1[i - 1] = 0.;
+        wv2[i - 1] = 0.;
+        if (i <= col) {
+            double t1 = 0., t2 = 0.;
+            LB_UNROLL
+            for (int j = 1; j <= n; ++j) {
+                if (j <= nsub) {
+                    int k = ind[j - 1];
+                    t1 += LB_WYK(k, i) * d[j - 1];
+                    t2 += LB_WSK(k, i) * d[j - 1];
+                }
+            }
+            wv1[i - 1] = t1;
+            wv2[i - 1] = theta * t2;
         }
-        wv[i - 1] = t1;
-        wv[col + i - 1] = theta * t2;
-        pointr = pointr % m + 1;
     }
-    int info = lb_trsl(w->wn, 2 * m, 2 * col, wv, 11);
+    int info = lb_trsl_k(w, col, wv1, wv2, 11);
     if (info) return info;
-    for (int i = 0; i < col; ++i) wv[i] = -wv[i];
-    info = lb_trsl(w->wn, 2 * m, 2 * col, wv, 1);
+    LB_UNROLL
+    for (int i = 0; i < m; ++i)
+        if (i < col) wv1[i] = -wv1[i];
+    info = lb_trsl_k(w, col, wv1, wv2, 1);
     if (info) return info;
-    pointr = head;
-    for (int jy = 1; jy <= col; ++jy) {
-        int js = col + jy;
-        for (int i = 1; i <= nsub; ++i) {
-            int k = ind[(i) - 1];
-            d[(i) - 1] = d[(i) - 1] + LB_WY(k, pointr) * wv[jy - 1] / theta + LB_WS(k, pointr) * wv[js - 1];
+    LB_UNROLL
+    for (int jy = 1; jy <= m; ++jy) {
+        if (jy <= col) {
+            LB_UNROLL
+            for (int i = 1; i <= n; ++i) {
+                if (i <= nsub) {
+                    int k = ind[i - 1];
+                    d[i - 1] = d[i - 1] + LB_WYK(k, jy) * wv1[jy - 1] / theta
+                             + LB_WSK(k, jy) * wv2[jy - 1];
+                }
+            }
         }
-        pointr = pointr % m + 1;
     }
     double rt = 1. / theta;
-    for (int i = 0; i < nsub; i++) w->r[i] = rt * w->r[i];
+    LB_UNROLL
+    for (int i = 0; i < n; i++)
+        if (i < nsub) w->r[i] = rt * w->r[i];
     int iword = 0;
-    for (int i = 0; i < n; i++) w->xp[i] = w->z[i];
-    for (int i = 1; i <= nsub; ++i) {
-        int k = ind[(i) - 1];
-        double dk = d[(i) - 1], xk = x[(k) - 1];
-        xk = lb_max(l[(k) - 1], xk + dk);
-        x[(k) - 1] = lb_min(u[(k) - 1], xk);
-        if (x[(k) - 1] == l[(k) - 1] || x[(k) - 1] == u[(k) - 1]) iword = 1;
+    xp[0] = w->z[0];
+    xp[1] = w->z[1];
+    LB_UNROLL
+    for (int i = 1; i <= n; ++i) {
+        if (i <= nsub) {
+            int k = ind[i - 1];
+            double dk = d[i - 1], xk = LB_AT(x, k);
+            xk = lb_max(LB_AT(l, k), xk + dk);
+            LB_PUT(x, k, lb_min(LB_AT(u, k), xk));
+            if (LB_AT(x, k) == LB_AT(l, k) || LB_AT(x, k) == LB_AT(u, k)) iword = 1;
+        }
     }
     if (iword == 0) return 0;
     double dd_p = 0.;
-    for (int i = 1; i <= n; ++i) dd_p += (x[(i) - 1] - xx[(i) - 1]) * gg[(i) - 1];
+    LB_UNROLL
+    for (int i = 1; i <= n; ++i) dd_p += (x[i - 1] - xx[i - 1]) * gg[i - 1];
     if (dd_p > 0.) {
-        for (int i = 0; i < n; i++) w->z[i] = w->xp[i];
+        w->z[0] = xp[0];
+        w->z[1] = xp[1];
     } else {
         return 0;
     }
     double alpha = 1., t1 = alpha;
     int ibd = 0;
-    for (int i = 1; i <= nsub; ++i) {
-        int k = ind[(i) - 1];
-        double dk = d[(i) - 1];
-        if (dk < 0.) {
-            double t2 = l[(k) - 1] - x[(k) - 1];
-            if (t2 >= 0.) t1 = 0.;
-            else if (dk * alpha < t2) t1 = t2 / dk;
-        } else if (dk > 0.) {
-            double t2 = u[(k) - 1] - x[(k) - 1];
-            if (t2 <= 0.) t1 = 0.;
-            else if (dk * alpha > t2) t1 = t2 / dk;
+    LB_UNROLL
+    for (int i = 1; i <= n; ++i) {
+        if (i <= nsub) {
+            int k = ind[i - 1];
+            double dk = d[i - 1];
+            if (dk < 0.) {
+                double t2 = LB_AT(l, k) - LB_AT(x, k);
+                if (t2 >= 0.) t1 = 0.;
+                else if (dk * alpha < t2) t1 = t2 / dk;
+            } else if (dk > 0.) {
+                double t2 = LB_AT(u, k) - LB_AT(x, k);
+                if (t2 <= 0.) t1 = 0.;
+                else if (dk * alpha > t2) t1 = t2 / dk;
+            }
+            if (t1 < alpha) {
+                alpha = t1;
+                ibd = i;
+            }
         }
-        if (t1 < alpha) { alpha = t1; ibd = i; }
     }
     if (alpha < 1.) {
-        double dk = d[(ibd) - 1];
-        int k = ind[(ibd) - 1];
-        if (dk > 0.) { x[(k) - 1] = u[(k) - 1]; d[(ibd) - 1] = 0.; }
-        else if (dk < 0.) { x[(k) - 1] = l[(k) - 1]; d[(ibd) - 1] = 0.; }
+        double dk = LB_AT(d, ibd);
+        int k = LB_AT(ind, ibd);
+        if (dk > 0.) {
+            LB_PUT(x, k, LB_AT(u, k));
+            LB_PUT(d, ibd, 0.);
+        } else if (dk < 0.) {
+            LB_PUT(x, k, LB_AT(l, k));
+            LB_PUT(d, ibd, 0.);
+        }
     }
-    for (int i = 1; i <= nsub; ++i) {
-        int k = ind[(i) - 1];
-        x[(k) - 1] += alpha * d[(i) - 1];
+    LB_UNROLL
+    for (int i = 1; i <= n; ++i) {
+        if (i <= nsub) {
+            int k = ind[i - 1];
+            LB_PUT(x, k, LB_AT(x, k) + alpha * d[i - 1]);
+        }
     }
     return 0;
 }
 
-/* subalgorithms.c matupd */
-FSX_HD void lb_matupd(fsx_lb *w, int *itail, int iupdat, int *col, int *head,
-                      double *theta, double rr, double dr, double stp, double dtd)
+/* subalgorithms.c matupd: append the pair (s, y) = (d, r), dropping the
+ * oldest when m are held (the reference advances its ring-buffer head and
+ * tail; here the pairs move one column down), and update S'Y, S'S. */
+FSX_HD void lb_matupd(fsx_lb *w, int iupdat, int *col, double *theta, double rr, double dr,
+                      double stp, double dtd)
 {
     const int n = FSX_LB_N, m = FSX_LB_M;
     if (iupdat <= m) {
         *col = iupdat;
-        *itail = (*head + iupdat - 2) % m + 1;
     } else {
-        *itail = *itail % m + 1;
-        *head = *head % m + 1;
-    }
-    for (int i = 0; i < n; i++) {
-        LB_WS(i + 1, *itail) = w->d[i];
-        LB_WY(i + 1, *itail) = w->r[i];
-    }
-    *theta = rr / dr;
-    if (iupdat > m) {
-        for (int j = 1; j <= *col - 1; ++j) {
-            for (int k = 0; k < j; k++) LB_SS(1 + k, j) = LB_SS(2 + k, j + 1);
-            for (int k = 0; k < *col - j; k++) LB_SY(j + k, j) = LB_SY(j + 1 + k, j + 1);
+        LB_UNROLL
+        for (int j = 1; j <= m - 1; ++j) {
+            LB_UNROLL
+            for (int i = 1; i <= n; ++i) {
+                LB_WS(i, j) = LB_WS(i, j + 1);
+                LB_WY(i, j) = LB_WY(i, j + 1);
+            }
         }
     }
-    int pointr = *head;
-    for (int j = 1; j <= *col - 1; ++j) {
-        LB_SY(*col, j) = lb_dot(n, w->d, &LB_WY(1, pointr));
-        LB_SS(j, *col) = lb_dot(n, &LB_WS(1, pointr), w->d);
-        pointr = pointr % m + 1;
+    const int c = *col;
+    *theta = rr / dr;
+    /* when iupdat > m, col = m: move the old parts of S'S and S'Y */
+    if (iupdat > m) {
+        LB_UNROLL
+        for (int j = 1; j <= m - 1; ++j) {
+            LB_UNROLL
+            for (int k = 0; k < j; k++) LB_SS(1 + k, j) = LB_SS(2 + k, j + 1);
+            LB_UNROLL
+            for (int k = 0; k < m - j; k++) LB_SY(j + k, j) = LB_SY(j + 1 + k, j + 1);
+        }
     }
-    if (stp == 1.) LB_SS(*col, *col) = dtd;
-    else LB_SS(*col, *col) = stp * stp * dtd;
-    LB_SY(*col, *col) = dr;
+    LB_UNROLL
+    for (int cc = 1; cc <= m; ++cc) {
+        if (cc == c) {
+            LB_UNROLL
+            for (int i = 1; i <= n; ++i) {
+                LB_WS(i, cc) = w->d[i - 1];
+                LB_WY(i, cc) = w->r[i - 1];
+            }
+            LB_UNROLL
+            for (int j = 1; j <= cc - 1; ++j) {
+                double sy = 0., ss = 0.;
+                LB_UNROLL
+                for (int i = 1; i <= n; ++i) sy = sy + w->d[i - 1] * LB_WY(i, j);
+                LB_UNROLL
+                for (int i = 1; i <= n; ++i) ss = ss + LB_WS(i, j) * w->d[i - 1];
+                LB_SY(cc, j) = sy;
+                LB_SS(j, cc) = ss;
+            }
+            if (stp == 1.) LB_SS(cc, cc) = dtd;
+            else LB_SS(cc, cc) = stp * stp * dtd;
+            LB_SY(cc, cc) = dr;
+        }
+    }
 }
 
-/* linesearch.c dcstep */
-FSX_HD void lb_dcstep(double *stx, double *fx, double *dx, double *sty, double *fy,
-                      double *dy, double *stp, double fp, double dp, int *brackt,
+/* linesearch.c dcstep.  The in/out arguments are copied into locals and
+ * back, so that merged branches never select between their addresses. */
+FSX_HD void lb_dcstep(double *stx_, double *fx_, double *dx_, double *sty_, double *fy_,
+                      double *dy_, double *stp_, double fp, double dp, int *brackt_,
                       double stpmin, double stpmax)
 {
-    double sgnd = dp * (*dx / lb_abs(*dx));
+    double stx = *stx_, fx = *fx_, dx = *dx_, sty = *sty_, fy = *fy_, dy = *dy_;
+    double stp = *stp_;
+    int brackt = *brackt_;
+    double sgnd = dp * (dx / lb_abs(dx));
     double stpf, stpc, stpq, theta, s, gamma, p, q, r, tt;
-    if (fp > *fx) {
-        theta = (*fx - fp) * 3. / (*stp - *stx) + *dx + dp;
-        s = lb_max(lb_max(lb_abs(theta), lb_abs(*dx)), lb_abs(dp));
+    if (fp > fx) {
+        theta = (fx - fp) * 3. / (stp - stx) + dx + dp;
+        s = lb_max(lb_max(lb_abs(theta), lb_abs(dx)), lb_abs(dp));
         tt = theta / s;
-        gamma = s * sqrt(tt * tt - *dx / s * (dp / s));
-        if (*stp < *stx) gamma = -gamma;
-        p = gamma - *dx + theta;
-        q = gamma - *dx + gamma + dp;
+        gamma = s * sqrt(tt * tt - dx / s * (dp / s));
+        if (stp < stx) gamma = -gamma;
+        p = gamma - dx + theta;
+        q = gamma - dx + gamma + dp;
         r = p / q;
-        stpc = *stx + r * (*stp - *stx);
-        stpq = *stx + *dx / ((*fx - fp) / (*stp - *stx) + *dx) / 2. * (*stp - *stx);
-        if (lb_abs(stpc - *stx) < lb_abs(stpq - *stx)) stpf = stpc;
+        stpc = stx + r * (stp - stx);
+        stpq = stx + dx / ((fx - fp) / (stp - stx) + dx) / 2. * (stp - stx);
+        if (lb_abs(stpc - stx) < lb_abs(stpq - stx)) stpf = stpc;
         else stpf = stpc + (stpq - stpc) / 2.;
-        *brackt = 1;
+        brackt = 1;
     } else if (sgnd < 0.) {
-        theta = (*fx - fp) * 3. / (*stp - *stx) + *dx + dp;
-        s = lb_max(lb_max(lb_abs(theta), lb_abs(*dx)), lb_abs(dp));
+        theta = (fx - fp) * 3. / (stp - stx) + dx + dp;
+        s = lb_max(lb_max(lb_abs(theta), lb_abs(dx)), lb_abs(dp));
         tt = theta / s;
-        gamma = s * sqrt(tt * tt - *dx / s * (dp / s));
-        if (*stp > *stx) gamma = -gamma;
+        gamma = s * sqrt(tt * tt - dx / s * (dp / s));
+        if (stp > stx) gamma = -gamma;
         p = gamma - dp + theta;
-        q = gamma - dp + gamma + *dx;
+        q = gamma - dp + gamma + dx;
         r = p / q;
-        stpc = *stp + r * (*stx - *stp);
-        stpq = *stp + dp / (dp - *dx) * (*stx - *stp);
-        if (lb_abs(stpc - *stp) > lb_abs(stpq - *stp)) stpf = stpc;
+        stpc = stp + r * (stx - stp);
+        stpq = stp + dp / (dp - dx) * (stx - stp);
+        if (lb_abs(stpc - stp) > lb_abs(stpq - stp)) stpf = stpc;
         else stpf = stpq;
-        *brackt = 1;
-    } else if (lb_abs(dp) < lb_abs(*dx)) {
-        theta = (*fx - fp) * 3. / (*stp - *stx) + *dx + dp;
-        s = lb_max(lb_max(lb_abs(theta), lb_abs(*dx)), lb_abs(dp));
+        brackt = 1;
+    } else if (lb_abs(dp) < lb_abs(dx)) {
+        theta = (fx - fp) * 3. / (stp - stx) + dx + dp;
+        s = lb_max(lb_max(lb_abs(theta), lb_abs(dx)), lb_abs(dp));
         tt = theta / s;
-        gamma = s * sqrt(lb_max(0., tt * tt - *dx / s * (dp / s)));
-        if (*stp > *stx) gamma = -gamma;
+        gamma = s * sqrt(lb_max(0., tt * tt - dx / s * (dp / s)));
+        if (stp > stx) gamma = -gamma;
         p = gamma - dp + theta;
-        q = gamma + (*dx - dp) + gamma;
+        q = gamma + (dx - dp) + gamma;
         r = p / q;
-        if (r < 0. && gamma != 0.) stpc = *stp + r * (*stx - *stp);
-        else if (*stp > *stx) stpc = stpmax;
+        if (r < 0. && gamma != 0.) stpc = stp + r * (stx - stp);
+        else if (stp > stx) stpc = stpmax;
         else stpc = stpmin;
-        stpq = *stp + dp / (dp - *dx) * (*stx - *stp);
-        if (*brackt) {
-            if (lb_abs(stpc - *stp) < lb_abs(stpq - *stp)) stpf = stpc;
+        stpq = stp + dp / (dp - dx) * (stx - stp);
+        if (brackt) {
+            if (lb_abs(stpc - stp) < lb_abs(stpq - stp)) stpf = stpc;
             else stpf = stpq;
-            if (*stp > *stx) stpf = lb_min(*stp + (*sty - *stp) * .66, stpf);
-            else stpf = lb_max(*stp + (*sty - *stp) * .66, stpf);
+            if (stp > stx) stpf = lb_min(stp + (sty - stp) * .66, stpf);
+            else stpf = lb_max(stp + (sty - stp) * .66, stpf);
         } else {
-            if (lb_abs(stpc - *stp) > lb_abs(stpq - *stp)) stpf = stpc;
+            if (lb_abs(stpc - stp) > lb_abs(stpq - stp)) stpf = stpc;
             else stpf = stpq;
             stpf = lb_min(stpmax, stpf);
             stpf = lb_max(stpmin, stpf);
         }
     } else {
-        if (*brackt) {
-            theta = (fp - *fy) * 3. / (*sty - *stp) + *dy + dp;
-            s = lb_max(lb_max(lb_abs(theta), lb_abs(*dy)), lb_abs(dp));
+        if (brackt) {
+            theta = (fp - fy) * 3. / (sty - stp) + dy + dp;
+            s = lb_max(lb_max(lb_abs(theta), lb_abs(dy)), lb_abs(dp));
             tt = theta / s;
-            gamma = s * sqrt(tt * tt - *dy / s * (dp / s));
-            if (*stp > *sty) gamma = -gamma;
+            gamma = s * sqrt(tt * tt - dy / s * (dp / s));
+            if (stp > sty) gamma = -gamma;
             p = gamma - dp + theta;
-            q = gamma - dp + gamma + *dy;
+            q = gamma - dp + gamma + dy;
             r = p / q;
-            stpc = *stp + r * (*sty - *stp);
+            stpc = stp + r * (sty - stp);
             stpf = stpc;
-        } else if (*stp > *stx) {
+        } else if (stp > stx) {
             stpf = stpmax;
         } else {
             stpf = stpmin;
         }
     }
-    if (fp > *fx) {
-        *sty = *stp; *fy = fp; *dy = dp;
+    if (fp > fx) {
+        sty = stp; fy = fp; dy = dp;
     } else {
-        if (sgnd < 0.) { *sty = *stx; *fy = *fx; *dy = *dx; }
-        *stx = *stp; *fx = fp; *dx = dp;
+        if (sgnd < 0.) { sty = stx; fy = fx; dy = dx; }
+        stx = stp; fx = fp; dx = dp;
     }
-    *stp = stpf;
+    stp = stpf;
+    *stx_ = stx; *fx_ = fx; *dx_ = dx;
+    *sty_ = sty; *fy_ = fy; *dy_ = dy;
+    *stp_ = stp;
+    *brackt_ = brackt;
 }
 
 typedef struct fsx_ls {
@@ -991,22 +1371,31 @@ FSX_HD void lb_dcsrch(double f, double g, double *stp, double stpmin, double stp
     if (*stp == stpmin && (f > ftest || g >= s->gtest)) *task = FSX_T_W_STPMIN;
     if (f <= ftest && lb_abs(g) <= gtol * (-s->ginit)) *task = FSX_T_CONVERGENCE;
     if ((*task >= 100 && *task <= 110) || (*task >= 20 && *task <= 25)) return;
-    if (s->stage == 1 && f <= s->fx && f > ftest) {
-        double fm = f - *stp * s->gtest;
-        double fxm = s->fx - s->stx * s->gtest;
-        double fym = s->fy - s->sty * s->gtest;
-        double gm = g - s->gtest;
-        double gxm = s->gx - s->gtest;
-        double gym = s->gy - s->gtest;
-        lb_dcstep(&s->stx, &fxm, &gxm, &s->sty, &fym, &gym, stp, fm, gm, &s->brackt,
-                  s->stmin, s->stmax);
-        s->fx = fxm + s->stx * s->gtest;
-        s->fy = fym + s->sty * s->gtest;
-        s->gx = gxm + s->gtest;
-        s->gy = gym + s->gtest;
+    /* one dcstep, on the modified function (stage 1, lower function value
+     * with sufficient decrease not yet met) or on f itself */
+    const int modf = s->stage == 1 && f <= s->fx && f > ftest;
+    double fv, gv, fxv, fyv, gxv, gyv;
+    if (modf) {
+        fv = f - *stp * s->gtest;
+        fxv = s->fx - s->stx * s->gtest;
+        fyv = s->fy - s->sty * s->gtest;
+        gv = g - s->gtest;
+        gxv = s->gx - s->gtest;
+        gyv = s->gy - s->gtest;
     } else {
-        lb_dcstep(&s->stx, &s->fx, &s->gx, &s->sty, &s->fy, &s->gy, stp, f, g, &s->brackt,
-                  s->stmin, s->stmax);
+        fv = f; fxv = s->fx; fyv = s->fy;
+        gv = g; gxv = s->gx; gyv = s->gy;
+    }
+    lb_dcstep(&s->stx, &fxv, &gxv, &s->sty, &fyv, &gyv, stp, fv, gv, &s->brackt, s->stmin,
+              s->stmax);
+    if (modf) {
+        s->fx = fxv + s->stx * s->gtest;
+        s->fy = fyv + s->sty * s->gtest;
+        s->gx = gxv + s->gtest;
+        s->gy = gyv + s->gtest;
+    } else {
+        s->fx = fxv; s->fy = fyv;
+        s->gx = gxv; s->gy = gyv;
     }
     if (s->brackt) {
         if (lb_abs(s->sty - s->stx) >= s->width1 * .66) *stp = s->stx + (s->sty - s->stx) * .5;
@@ -1030,14 +1419,30 @@ FSX_HD void lb_dcsrch(double f, double g, double *stp, double stpmin, double stp
 
 FSX_HD void lb_zero(fsx_lb *w)
 {
-    for (int i = 0; i < FSX_LB_N * FSX_LB_M; i++) { w->ws[i] = 0.0; w->wy[i] = 0.0; }
-    for (int i = 0; i < FSX_LB_M * FSX_LB_M; i++) { w->sy[i] = 0.0; w->ss[i] = 0.0; w->wt[i] = 0.0; }
-    for (int i = 0; i < 4 * FSX_LB_M * FSX_LB_M; i++) { w->wn[i] = 0.0; w->snd[i] = 0.0; }
+    LB_UNROLL
     for (int i = 0; i < FSX_LB_N; i++) {
-        w->z[i] = w->r[i] = w->d[i] = w->t[i] = w->xp[i] = 0.0;
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) { w->ws[i][j] = 0.0; w->wy[i][j] = 0.0; }
+    }
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++) {
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) {
+            w->sy[i][j] = 0.0; w->ss[i][j] = 0.0; w->wt[i][j] = 0.0;
+            w->wn11[i][j] = 0.0; w->wn12[i][j] = 0.0; w->wn22[i][j] = 0.0;
+        }
+    }
+    LB_UNROLL
+    for (int i = 0; i < 2 * FSX_LB_M; i++) {
+        w->wc[i] = 0.0;
+        LB_UNROLL
+        for (int j = 0; j < 2 * FSX_LB_M; j++) w->wn1[i][j] = 0.0;
+    }
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_N; i++) {
+        w->z[i] = w->r[i] = w->d[i] = w->t[i] = 0.0;
         w->index[i] = w->iwhere[i] = w->indx2[i] = 0;
     }
-    for (int i = 0; i < 8 * FSX_LB_M; i++) w->wa[i] = 0.0;
 }
 
 /* The objective evaluation requested by the wrapper on every FG task
@@ -1070,7 +1475,7 @@ typedef struct fsx_lbrc {
     double x[FSX_LB_N], g[FSX_LB_N], f;
     double l[FSX_LB_N], u[FSX_LB_N];
     double theta, fold, gd, stpmx, sbgnrm, stp, gdold, dtd, tol, pgtol;
-    int col, head, iupdat, updatd, iback, itail, iter, nseg, nfree, ifun, nenter, ileave;
+    int col, iupdat, updatd, iback, iter, nseg, nfree, ifun, nenter, ileave;
     int wrk, csave, nev, task, phase;
 } fsx_lbrc;
 
@@ -1087,7 +1492,7 @@ FSX_HD void fsx_lb_start(fsx_lbrc *s, const fsx_fitpar *p, double x0, double x1)
     s->u[0] = p->hi0; s->u[1] = p->hi1;
     s->x[0] = x0; s->x[1] = x1;
     s->g[0] = 0.0; s->g[1] = 0.0; s->f = 0.0;
-    s->col = 0; s->head = 1; s->iupdat = 0; s->updatd = 0; s->iback = 0; s->itail = 0;
+    s->col = 0; s->iupdat = 0; s->updatd = 0; s->iback = 0;
     s->iter = 0; s->nseg = 0; s->nfree = FSX_LB_N; s->ifun = 0; s->nenter = 0; s->ileave = 0;
     s->theta = 1.; s->fold = 0.; s->gd = 0.; s->stpmx = 0.; s->sbgnrm = 0.; s->stp = 0.;
     s->gdold = 0.; s->dtd = 0.;
@@ -1130,14 +1535,14 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
     } while (0)
 #define LB_RESET()                                                              \
     do {                                                                        \
-        info = 0; s->col = 0; s->head = 1; s->theta = 1.;                       \
+        info = 0; s->col = 0; s->theta = 1.;                                    \
         s->iupdat = 0; s->updatd = 0;                                           \
     } while (0)
     /* the first evaluation (at the projected x0) is done */
     LB_PROJGR(s->sbgnrm);
     if (s->sbgnrm <= s->pgtol) { s->task = FSX_T_CONV_GRAD; goto done; }
     for (;;) {
-        info = lb_cauchy(w, x, l, u, g, s->theta, s->col, s->head, s->sbgnrm, &s->nseg);
+        info = lb_cauchy(w, x, l, u, g, s->theta, s->col, s->sbgnrm, &s->nseg);
         if (info) {
             LB_RESET();
             continue;
@@ -1146,13 +1551,13 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
         if (!(s->nfree == 0 || s->col == 0)) {
             if (s->wrk)
                 info = lb_formk(w, s->nfree, s->nenter, s->ileave, s->iupdat, s->updatd,
-                                s->theta, s->col, s->head);
+                                s->theta, s->col);
             if (info) {
                 LB_RESET();
                 continue;
             }
-            info = lb_cmprlb(w, x, g, s->theta, s->col, s->head, s->nfree);
-            if (!info) info = lb_subsm(w, s->nfree, l, u, x, g, s->theta, s->col, s->head);
+            info = lb_cmprlb(w, x, g, s->theta, s->col, s->nfree);
+            if (!info) info = lb_subsm(w, s->nfree, l, u, x, g, s->theta, s->col);
             if (info) {
                 LB_RESET();
                 continue;
@@ -1160,7 +1565,7 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
         }
         for (int i = 0; i < n; i++) w->d[i] = w->z[i] - x[i];
         /* lnsrlb entry (every variable boxed) */
-        s->dtd = lb_dot(n, w->d, w->d);
+        s->dtd = lb_dot2(w->d, w->d);
         if (s->iter == 0) {
             s->stpmx = 1.;
         } else {
@@ -1186,7 +1591,7 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
         s->csave = FSX_T_START;
         restart = 0;
         for (;;) {
-            s->gd = lb_dot(n, g, w->d);
+            s->gd = lb_dot2(g, w->d);
             skip_search = 0;
             if (s->ifun == 0) {
                 s->gdold = s->gd;
@@ -1235,7 +1640,7 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
         ddum = lb_max(lb_max(lb_abs(s->fold), lb_abs(s->f)), 1.);
         if (s->fold - s->f <= s->tol * ddum) { s->task = FSX_T_CONV_F; goto done; }
         for (int i = 0; i < n; i++) w->r[i] = g[i] - w->r[i];
-        rr = lb_dot(n, w->r, w->r);
+        rr = lb_dot2(w->r, w->r);
         if (s->stp == 1.) {
             dr = s->gd - s->gdold;
             ddum = -s->gdold;
@@ -1250,8 +1655,7 @@ FSX_HD int fsx_lb_resume(fsx_lbrc *s)
         }
         s->updatd = 1;
         ++s->iupdat;
-        lb_matupd(w, &s->itail, s->iupdat, &s->col, &s->head, &s->theta, rr, dr, s->stp,
-                  s->dtd);
+        lb_matupd(w, s->iupdat, &s->col, &s->theta, rr, dr, s->stp, s->dtd);
         if (lb_formt(w, s->col, s->theta)) LB_RESET();
     }
 #undef LB_PROJGR

@@ -1125,8 +1125,13 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, ntiles, nrows, Q, L, nb, na, ng, zn);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[4], s));
-    /* per-pose fits over a pose queue */
-    long lanes = (long)g_ncu * 4 * 64 * env_int("FSX_FIT_WAVES_PER_SIMD", 2, 1, 16);
+    /* per-pose fits over a pose queue: as many lanes as are resident at once
+     * (the fit state lives in registers, one 256-thread block per CU) */
+    int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
+    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, 0)
+                             != hipSuccess || fit_bpc < 1))
+        fit_bpc = 1;
+    long lanes = (long)g_ncu * 256 * fit_bpc;
     if (lanes > n) lanes = ((long)n + 255) / 256 * 256;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_NEXT], 1L, (int)lanes);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_TALLY], 4L, 0);

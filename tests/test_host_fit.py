@@ -32,3 +32,38 @@ def test_host_fit_bit_identical_to_oracle(F, orc, bounds):
         n_bound += c1 in bounds[0] or c2 in bounds[1]
     # the set exercises fits that end on a bound as well as interior ones
     assert 0 < n_bound < len(r["cross"])
+
+
+def test_host_fit_stress_bit_identical(F, orc):
+    """Harder fits for the fixed-size L-BFGS-B: each pose profile as is,
+    with the water channel set to zero (the c2 gradient is then exactly 0,
+    which reaches cauchy's early return and the zero-gradient branches), and
+    with rescaled channels; five bound boxes, narrow ones included, so that
+    fits end on bounds, take up to tens of evaluations and cycle the
+    correction pairs (more than m = 3 updates)."""
+    pb = build_problem(300, 200, 8, 0.3, 25, np.arange(15.0, 35.0), 120, seed=23)
+    P0 = O.Params(pb["a"], pb["rm"], pb["peak"])
+    r = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                              P0, pb["qvals"], pb["zvals"], pb["L"], want_cross=True)
+    rng = np.random.default_rng(1)
+    cts = []
+    for ct in r["cross"]:
+        cts.append(ct)
+        w0 = ct.copy()
+        w0[[2, 4, 5]] = 0.0
+        cts.append(w0)
+        s = ct.copy()
+        s[[1, 3, 4]] *= rng.uniform(0.7, 1.3)
+        s[[2, 4, 5]] *= rng.uniform(0.0, 3.0)
+        cts.append(s)
+    boxes = [((0.96, 1.04), (-2.0, 4.0)), ((0.95, 1.05), (0.0, 4.0)), ((0.99, 1.01), (0.5, 0.6)),
+             ((0.9, 1.2), (-5.0, 10.0)), ((1.0, 1.001), (0.0, 4.0))]
+    nev = []
+    for c1b, c2b in boxes:
+        P = O.Params(pb["a"], pb["rm"], pb["peak"], c1=c1b, c2=c2b)
+        for i, ct in enumerate(cts):
+            sc, c1, c2, ne, cr = O.fit_one(ct, pb["qvals"], P)
+            got = F.fit_host(cr, pb["qvals"], pb["a"], P.s.mult, c1=c1b, c2=c2b)
+            assert (got[0], got[1], got[2], got[3]) == (sc, c1, c2, ne), (c1b, c2b, i)
+            nev.append(ne)
+    assert max(nev) > 20 and np.mean(nev) > 4
