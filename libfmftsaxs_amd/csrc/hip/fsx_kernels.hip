@@ -373,26 +373,25 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
 }
 
 /* ---- per-pose cross terms, one m-chunk per launch --------------------------
- * Work item = (q, bucket), buckets ordered (beta2, gamma2, z) so that the
- * items in flight at one time share q (the receptor chunk table R[c][q][*]
- * stays in each XCD's L2) and a handful of ligand rows.  One wavefront per
- * block walks items grid-stride.  Per item:
- *   U^Y_e = sum_{l1 >= |m|} conj(T^{|m|}_{l l1}(q z)) B^Y_{(l1, m)}   (LDS)
- * then per pose 2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e]) over the
- * chunk's entries e = (l, m), added into acc[(s*Q + q)*6 + k]. */
-#define FSX_CROSS_BLOCK 256
-/* Work tile = (q, beta2, gamma2) with all z inside: the ligand row chunk
- * (3 * nc entries) is staged in LDS once per tile, the packed translation
- * block of each (z, q) once per z (TST; from global memory when it does not
- * fit), and each wavefront prefetches the receptor row of its first pose
- * before U is formed.  Then per non-empty bucket (z, beta2, gamma2):
- *   U^Y_e = sum_{l1 >= |m|} conj(T^{|m|}_{l l1}(q z)) B^Y_{(l1, m)}   (one entry per thread)
- * and per pose (one wavefront, J entries per lane)
+ * Work tile = (q, beta2, G consecutive gamma2) with all z inside.  The G
+ * ligand row chunks (3 * nc entries each) are staged in LDS once per tile;
+ * the packed translation block of each (z, q) is staged once per z (TST;
+ * read from global memory when it does not fit) and serves the G rows; each
+ * wavefront prefetches the receptor row of its first pose before U is
+ * formed.  Per z with requested poses in the tile's G buckets
+ * (z, beta2, gamma2), which are adjacent in the bucket order:
+ *   U^Y_e = sum_{l1 >= |m|} conj(T^{|m|}_{l l1}(q z)) B^Y_{(l1, m)}
+ * (one entry e per thread, for each of the G rows), then per pose (one
+ * wavefront, J entries per lane)
  *   2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e])
  * over the chunk's entries e = (l, m), stored (first chunk) or added into
  * acc[(s*Q + q)*6 + k]. */
-template <int J, int TST, typename E>
-__global__ void __launch_bounds__(FSX_CROSS_BLOCK)
+#define FSX_CROSS_BLOCK 256
+/* FP32 tables: at most 96 VGPRs, so that five blocks of four wavefronts fit
+ * a CU (with fewer registers than 96 the kernel spills); FP64 tables: the
+ * LDS of three blocks fills a CU, no register bound */
+template <int J, int TST, int G, typename E>
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 5 : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int na, int ng,
@@ -402,21 +401,25 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
     extern __shared__ double smem_raw[];
     E *smem = reinterpret_cast<E *>(smem_raw);
-    E *sB = smem;            /* [3][nc] */
-    E *sU = smem + 3 * nc;   /* [3][nc] */
-    E *sT = smem + 6 * nc;   /* T block, transposed per |m| (TST) */
+    E *sB = smem;                /* [G][3][nc] */
+    E *sU = smem + 3 * G * nc;   /* [G][3][nc] */
+    E *sT = smem + 6 * G * nc;   /* T block, transposed per |m| (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
     const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
     const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
     (void)na;
-    /* per-thread U entry, per-lane pairing entries */
+    /* per-thread U entry (entries ordered by decreasing row length, so that
+     * the lanes of a wavefront run loops of nearly equal length), per-lane
+     * pairing entries */
+    const int *uperm = ent + 7 * K2 + Tn + 2;
     const bool uact = tid < nc;
-    int ul = 0, um = 0, uam = 0;
+    int ue = 0, ul = 0, um = 0, uam = 0;
     if (uact) {
-        ul = ent_l[cstart + tid];
-        um = ent_m[cstart + tid];
+        ue = uperm[cstart + tid];
+        ul = ent_l[cstart + ue];
+        um = ent_m[cstart + ue];
         uam = um < 0 ? -um : um;
     }
     const int ub0 = uact ? mstart[um + L] - cstart : 0;
@@ -433,25 +436,30 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
     }
     const int slot = lane >> 3;
     const bool writer = (lane & 7) == 0 && slot < 6;
-    const int nbg = nbeta * ng;
+    const int ngg = (ng + G - 1) / G;   /* gamma2 groups per beta2 */
+    const int nbg = nbeta * ngg;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int q = (int)(tile / nbg), bg = (int)(tile - (long)q * nbg);
-        const int b2 = bg / ng, g2 = bg - (bg / ng) * ng;
-        const E *Bq = Bc + (((long)q * nrows + bg) * 3) * nc;
+        const int q = (int)(tile / nbg), bgg = (int)(tile - (long)q * nbg);
+        const int b2 = bgg / ngg, g20 = (bgg - b2 * ngg) * G;
+        const int nr = ng - g20 < G ? ng - g20 : G;   /* rows of this tile */
+        const E *Bq = Bc + (((long)q * nrows + b2 * ng + g20) * 3) * nc;
         bool staged = false;
         for (int z = 0; z < zn; z++) {
-            const int key = (z * nbeta + b2) * ng + g2;
-            const int cnt = counts[key];
+            /* the tile's buckets are adjacent keys, their poses adjacent in recs */
+            const int key = (z * nbeta + b2) * ng + g20;
+            const int cnt0 = counts[key];
+            const int cnt = G == 2 && nr == 2 ? cnt0 + counts[key + 1] : cnt0;
             if (cnt == 0) continue;
             const int start = offs[key];
             const E *Tq = T + ((long)z * Q + q) * Tn;
-            /* pose loader: receptor row entries, twiddles, accumulator */
+            /* pose loader: receptor row entries, twiddles, accumulator, row */
             E rr[3][J];
             F wr[J], wi[J];
             double old = 0.0;
-            int s = 0;
+            int s = 0, prow = 0;
             auto load_pose = [&](int p) {
                 s = start + p;
+                prow = p < cnt0 ? 0 : 1;
                 int4 rec = recs[s];
                 const E *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
 #pragma unroll
@@ -471,7 +479,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
-                const int nb3 = 3 * nc;
+                const int nb3 = 3 * nc * nr;
                 for (int b0 = tid; b0 < nb3; b0 += 4 * FSX_CROSS_BLOCK) {
                     const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
                     const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
@@ -505,44 +513,58 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                 const E *Tr = (TST ? sT : Tq) + urow;
                 const E *bp = sB + ub0;
                 const int nk = L - uam + 1;
-                F ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+                F ur[G][3], ui[G][3];
+#pragma unroll
+                for (int r = 0; r < G; r++)
+#pragma unroll
+                    for (int Y = 0; Y < 3; Y++) ur[r][Y] = ui[r][Y] = 0;
                 for (int k = 0; k < nk; k++) {
                     E tv = Tr[k * ustep];
 #pragma unroll
-                    for (int Y = 0; Y < 3; Y++) {
-                        E bv = bp[Y * nc + k];
-                        /* conj(t) * b */
-                        ur[Y] += tv.re * bv.re + tv.im * bv.im;
-                        ui[Y] += tv.re * bv.im - tv.im * bv.re;
-                    }
+                    for (int r = 0; r < G; r++)
+#pragma unroll
+                        for (int Y = 0; Y < 3; Y++) {
+                            E bv = bp[(r * 3 + Y) * nc + k];
+                            /* conj(t) * b */
+                            ur[r][Y] += tv.re * bv.re + tv.im * bv.im;
+                            ui[r][Y] += tv.re * bv.im - tv.im * bv.re;
+                        }
                 }
 #pragma unroll
-                for (int Y = 0; Y < 3; Y++) {
-                    sU[Y * nc + tid].re = ur[Y];
-                    sU[Y * nc + tid].im = ui[Y];
-                }
+                for (int r = 0; r < G; r++)
+#pragma unroll
+                    for (int Y = 0; Y < 3; Y++) {
+                        sU[(r * 3 + Y) * nc + ue].re = ur[r][Y];
+                        sU[(r * 3 + Y) * nc + ue].im = ui[r][Y];
+                    }
             }
             __syncthreads();
             for (int p = wave; p < cnt; p += nwave) {
                 if (p != wave) load_pose(p);
+                const E *sUp = sU + (G == 2 ? prow * 3 * nc : 0);
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
                     if (ec < nc) {
                         E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
-                        E uv = sU[ec], ud = sU[nc + ec], uw = sU[2 * nc + ec];
+                        E uv = sUp[ec], ud = sUp[nc + ec], uw = sUp[2 * nc + ec];
                         F w_r = wr[j], w_i = wi[j];
                         /* twiddled receptor factor: w * R^X */
                         F vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
                         F dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
                         F xr = w_r * rw.re - w_i * rw.im, xi = w_r * rw.im + w_i * rw.re;
-                        va[0] += vr * uv.re - vi * uv.im;
-                        va[1] += (vr * ud.re - vi * ud.im) + (dr * uv.re - di * uv.im);
-                        va[2] += (vr * uw.re - vi * uw.im) + (xr * uv.re - xi * uv.im);
-                        va[3] += dr * ud.re - di * ud.im;
-                        va[4] += (dr * uw.re - di * uw.im) + (xr * ud.re - xi * ud.im);
-                        va[5] += xr * uw.re - xi * uw.im;
+                        /* Re of each product, accumulated term by term (one
+                         * fused multiply-add per term) */
+                        va[0] += vr * uv.re; va[0] -= vi * uv.im;
+                        va[1] += vr * ud.re; va[1] -= vi * ud.im;
+                        va[1] += dr * uv.re; va[1] -= di * uv.im;
+                        va[2] += vr * uw.re; va[2] -= vi * uw.im;
+                        va[2] += xr * uv.re; va[2] -= xi * uv.im;
+                        va[3] += dr * ud.re; va[3] -= di * ud.im;
+                        va[4] += dr * uw.re; va[4] -= di * uw.im;
+                        va[4] += xr * ud.re; va[4] -= xi * ud.im;
+                        va[5] += xr * uw.re; va[5] -= xi * uw.im;
                     }
                 }
                 /* the cross-lane sums run in FP64 */
@@ -845,21 +867,24 @@ static void launch_rotate(hipStream_t s, void *out, const cplx *srct, long nrot,
                        (const long *)g_buf[B_COFF], nb, ng, Q, L, gc);
 }
 
-template <typename E>
-static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
-                         int Tn, long grid, long ntiles, int nrows, int Q, int L, int nb, int na,
-                         int ng, int zn)
+template <typename E, int G>
+static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
+                           const long *coff, int Tn, long grid, int nrows, int Q, int L,
+                           int nb, int na, int ng, int zn)
 {
     const int tst = (long)Tn * (long)sizeof(E) <= 40 * 1024;
+    const long ntiles = (long)Q * nb * ((ng + G - 1) / G);
     for (int c = 0; c < C; c++) {
         if (csz[c] == 0) continue;
         int jn = (csz[c] + FSX_WAVE - 1) / FSX_WAVE;
         void (*kern)(double *, const E *, const E *, const E *, const double *, const int *,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
                      int, int, int, int, int);
-        if (tst) kern = jn <= 1 ? k_cross<1, 1, E> : jn <= 2 ? k_cross<2, 1, E> : k_cross<4, 1, E>;
-        else kern = jn <= 1 ? k_cross<1, 0, E> : jn <= 2 ? k_cross<2, 0, E> : k_cross<4, 0, E>;
-        size_t lds = sizeof(E) * (6 * (size_t)csz[c] + (tst ? Tn : 0));
+        if (tst)
+            kern = jn <= 1 ? k_cross<1, 1, G, E> : jn <= 2 ? k_cross<2, 1, G, E> : k_cross<4, 1, G, E>;
+        else
+            kern = jn <= 1 ? k_cross<1, 0, G, E> : jn <= 2 ? k_cross<2, 0, G, E> : k_cross<4, 0, G, E>;
+        size_t lds = sizeof(E) * (6 * (size_t)G * csz[c] + (tst ? Tn : 0));
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
         long g = grid;
@@ -879,6 +904,17 @@ static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, c
                            (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, na, ng, zn,
                            c == 0);
     }
+}
+
+/* G = gamma2 rows per tile: FSX_CROSS_ROWS, else 2 */
+template <typename E>
+static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
+                         int Tn, long grid, int nrows, int Q, int L, int nb, int na, int ng,
+                         int zn)
+{
+    const int G = env_int("FSX_CROSS_ROWS", 2, 1, 2);
+    if (G == 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
 }
 
 /* Entry tables of the m-major coefficient order used by the rotated tables
@@ -933,7 +969,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
      * would exceed 4 entries per lane (FSX_CROSS_CHUNKS overrides) */
     int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
     const int Tn = tpk_sq(K);
-    const size_t nent = 7 * (size_t)K2 + Tn + 2;
+    const size_t nent = 8 * (size_t)K2 + Tn + 2;
     int *ent = (int *)malloc(sizeof(int) * nent);
     long *coff = (long *)malloc(sizeof(long) * (N + 1));
     if (!ent || !coff) {
@@ -963,6 +999,20 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
             for (int l = am; l <= L; l++)
                 for (int l1 = am; l1 <= L; l1++)
                     tmap[i++] = tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am);
+    }
+    /* U-phase thread -> entry of each chunk: entries by increasing |m|
+     * (decreasing row length K - |m|), in entry order within one |m| */
+    {
+        int *uperm = ent + 7 * K2 + Tn + 2;
+        const int *em = ent + K2, *csz = ent + 4 * K2, *cst = ent + 6 * K2;
+        for (int c = 0; c < C; c++) {
+            int t = 0;
+            for (int am = 0; am <= L; am++)
+                for (int i = 0; i < csz[c]; i++) {
+                    const int m = em[cst[c] + i];
+                    if ((m < 0 ? -m : m) == am) uperm[cst[c] + t++] = i;
+                }
+        }
     }
     /* twiddle e^{-i m a2 2pi/na} per (a2, m): the same table entries the
      * reference's exact transform uses, gathered by (m a2) mod na */
@@ -1119,10 +1169,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[3], s));
     /* per-pose cross terms, one launch per m-chunk, accumulated in order */
-    const long ntiles = (long)Q * nb * ng;
     long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
-    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, ntiles, nrows, Q, L, nb, na, ng, zn);
-    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, ntiles, nrows, Q, L, nb, na, ng, zn);
+    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue: as many lanes as are resident at once

@@ -108,6 +108,31 @@ def test_extended_grid_vs_oracle(F):
     _cmp_results(pb, r_g, r_o)
 
 
+def test_large_L_chunked_vs_oracle(F):
+    """L = 25, the largest L of the workloads: 676 coefficients per row, so
+    the cross-term kernel runs as three m-chunk launches that accumulate in
+    place, and the translation blocks (6201 entries) are too large for LDS
+    and are read from global memory.  Both forms, at their bars."""
+    L = 25
+    zv = np.arange(30.0, 34.0)
+    pb = build_problem(500, 300, L, 0.6, 11, zv, 24, seed=25)
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L, want_cross=True)
+    gp = _gpu_problem(F, pb)
+    r_g = F.score(gp, pb["index"], want_cross=True)
+    assert r_g["stats"]["cross_chunks"] == 3
+    rel = _cmp_results(pb, r_g, r_o)
+    r_m = F.score(gp, pb["index"], want_cross=True, mixed=True)
+    assert r_m["stats"]["cross_chunks"] == 3
+    ok = ~np.isnan(r_o["score"])
+    relI = np.abs(_I(r_m["cross"][ok]) - _I(r_o["cross"][ok])) / np.abs(_I(r_o["cross"][ok]))
+    relc = np.abs(r_m["score"][ok] - r_o["score"][ok]) / r_o["score"][ok]
+    print(f"L=25: FP64 max rel I {rel:.2e}; mixed max rel I {relI.max():.2e}, chi {relc.max():.2e}")
+    assert relI.max() <= 1e-5 and relc.max() <= 1e-5
+    assert np.array_equal(np.argsort(r_m["score"][ok], kind="stable"),
+                          np.argsort(r_o["score"][ok], kind="stable"))
+
+
 def test_out_of_range_poses_untouched(F):
     L = 6
     zv = np.arange(20.0, 26.0)
