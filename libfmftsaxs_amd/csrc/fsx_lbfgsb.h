@@ -124,6 +124,22 @@ FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
         double vr[FSX_QB][6];
         const double *v = t->v + i0 * t->vqs;
         const long vks = t->vks, vqs = t->vqs;
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (vks == 1 && vqs == 6) {
+            /* pose-major profile (the fit kernel's accumulator): the six
+             * terms of one q are contiguous and 16-byte aligned, so they are
+             * read as three pairs */
+            const double2 *v2 = reinterpret_cast<const double2 *>(v);
+#pragma unroll
+            for (int u = 0; u < FSX_QB; u++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double2 p = u < nb ? v2[u * 3 + k] : make_double2(0.0, 0.0);
+                    vr[u][2 * k] = p.x;
+                    vr[u][2 * k + 1] = p.y;
+                }
+        } else
+#endif
 #ifdef __HIPCC__
 #pragma unroll
 #endif

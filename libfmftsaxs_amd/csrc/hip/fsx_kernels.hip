@@ -169,15 +169,6 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-/* value of v in lane k (k wave-uniform), through two scalar reads */
-__device__ __forceinline__ double lane_bcast(double v, int k)
-{
-    int2 w = *reinterpret_cast<int2 *>(&v);
-    w.x = __builtin_amdgcn_readlane(w.x, k);
-    w.y = __builtin_amdgcn_readlane(w.y, k);
-    return *reinterpret_cast<double *>(&w);
-}
-
 /* x mod n in [0, n) for |x| < 2^24, n >= 1: a float quotient estimate
  * (off by at most one) and one correction. */
 __device__ __forceinline__ int mod_small(int x, int n, float inv_n)
@@ -218,69 +209,116 @@ __global__ void k_twiddle(cplx *dst, const double *src_re, const double *src_im,
     dst[i] = v;
 }
 
-/* one block per (q, X, l, gamma chunk of gc): the 2l+1 source rows of l,
- * gamma in the chunk, are staged in LDS ([m1][gamma]); each wavefront then
- * forms out^X_{lm}(beta, gamma) for (beta, m) pairs, lanes over gamma. */
+/* One block per (q, X, l, chunk of FSX_ROT_BC betas, gamma chunk of gc): the
+ * 2l+1 source rows of l, gamma in the chunk, are staged in LDS
+ * ([m1][gamma]) together with the blocks d^l_{m m1}(beta) of the chunk's
+ * betas; the wavefronts then take (beta, pair of m) items and form
+ * out^X_{lm}(beta, gamma) = sum_m1 d^l_{m m1}(beta) src_{l m1}(gamma), lanes
+ * over gamma, m1 ascending.  The output goes to a staging table with rows
+ * innermost, per chunk c [q][X][entry][row] at coff[c] (lanes store
+ * consecutive rows); k_tab_transpose then writes the row-major table the
+ * cross-term kernel reads. */
+#define FSX_ROT_BC 2
 template <typename E>
 __global__ void __launch_bounds__(256)
 k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
          int nbeta, int ng, int Q, int L, int gc)
 {
 #pragma clang fp contract(fast)
-    extern __shared__ cplx sR[];   /* [2l+1][gc] */
+    extern __shared__ cplx sR[];   /* [2l+1][gc], then FSX_ROT_BC blocks [2l+1][2l+1] of d */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1;
     const int *ent_c = ent + 2 * K2, *ent_ec = ent + 3 * K2, *csize = ent + 4 * K2;
     const int *mstart = ent + 5 * K2;
     const int nrows = nbeta * ng;
     const int lane = threadIdx.x % FSX_WAVE, wave = threadIdx.x / FSX_WAVE;
     const int nwave = blockDim.x / FSX_WAVE;
-    const int nch = (ng + gc - 1) / gc;
+    const int nch = (ng + gc - 1) / gc, nbc = (nbeta + FSX_ROT_BC - 1) / FSX_ROT_BC;
     int r = blockIdx.x;
     const int ch = r % nch; r /= nch;
+    const int bc = r % nbc; r /= nbc;
     const int l = r % K; r /= K;
     const int X = r % 3;
     const int q = r / 3;
     const int g0 = ch * gc, gn = (ng - g0) < gc ? (ng - g0) : gc;
-    const int w = 2 * l + 1;
+    const int b0 = bc * FSX_ROT_BC, nbh = nbeta - b0 < FSX_ROT_BC ? nbeta - b0 : FSX_ROT_BC;
+    const int w = 2 * l + 1, ww = w * w;
+    double *sD = reinterpret_cast<double *>(sR + w * gc);
     const cplx *src = srct + (((long)q * 3 + X) * K2 + l * (l + 1) - l) * ng + g0;
     for (int i = threadIdx.x; i < w * gc; i += blockDim.x) {
         int m1 = i / gc, g = i - m1 * gc;
         if (g < gn) sR[i] = src[(long)m1 * ng + g];
     }
+    for (int i = threadIdx.x; i < nbh * ww; i += blockDim.x) {
+        const int bb = i / ww, rem = i - bb * ww, mr = rem / w, k = rem - mr * w;
+        sD[i] = dW[(long)(b0 + bb) * K * N * N + ((long)l * N + (L - l + mr)) * N + (L - l + k)];
+    }
     __syncthreads();
-    for (int t = wave; t < nbeta * w; t += nwave) {
-        const int b = t / w, m = t - b * w - l;
-        const double *drow = dW + (long)b * K * N * N + ((long)l * N + (L + m)) * N + (L - l);
-        /* the row d^l_{m, -l..l}(beta): one element per lane, one load, then
-         * broadcast term by term (w <= 2L+1 <= 81 lanes over two registers) */
-        const double dk0 = lane < w ? drow[lane] : 0.0;
-        const double dk1 = lane + FSX_WAVE < w ? drow[lane + FSX_WAVE] : 0.0;
-        /* entry (l, m) in m-major order */
-        const int e = mstart[m + L] + (l - (m < 0 ? -m : m));
-        const int c = ent_c[e];
-        E *o = out + coff[c] + ent_ec[e];
-        const long cs = csize[c];
-        /* every lane runs the loop (the broadcasts need the whole
-         * wavefront); lanes past the chunk's end compute nothing */
+    const int np = (w + 1) / 2;   /* pairs of m */
+    for (int it = wave; it < nbh * np; it += nwave) {
+        const int bb = it / np, mi = 2 * (it - bb * np), b = b0 + bb;
+        const bool two = mi + 1 < w;
+        const double *d0 = sD + bb * ww + mi * w, *d1 = two ? d0 + w : d0;
+        E *o[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            /* entry (l, m) in m-major order */
+            const int m = (two ? mi + u : mi) - l;
+            const int e = mstart[m + L] + (l - (m < 0 ? -m : m));
+            const int c = ent_c[e];
+            const long cs = csize[c];
+            o[u] = out + coff[c] + ((long)(q * 3 + X) * cs + ent_ec[e]) * nrows + (long)b * ng + g0;
+        }
         for (int gw = 0; gw < gn; gw += FSX_WAVE) {
             const int g = gw + lane;
-            const bool on = g < gn;
-            double re = 0.0, im = 0.0;
-            for (int k = 0; k < w; k++) {
-                double dv = k < FSX_WAVE ? lane_bcast(dk0, k) : lane_bcast(dk1, k - FSX_WAVE);
-                if (on) {
-                    cplx v = sR[k * gc + g];
-                    re += dv * v.re;
-                    im += dv * v.im;
+            if (g < gn) {
+                double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 4
+                for (int k = 0; k < w; k++) {
+                    const cplx v = sR[k * gc + g];
+                    const double a0 = d0[k], a1 = d1[k];
+                    r0 += a0 * v.re;
+                    i0 += a0 * v.im;
+                    r1 += a1 * v.re;
+                    i1 += a1 * v.im;
                 }
-            }
-            if (on) {
-                E v;
-                v.re = re;
-                v.im = im;
-                o[(((long)q * nrows + (long)b * ng + g0 + g) * 3 + X) * cs] = v;
+                E v0, v1;
+                v0.re = r0;
+                v0.im = i0;
+                v1.re = r1;
+                v1.im = i1;
+                o[0][g] = v0;
+                if (two) o[1][g] = v1;
             }
         }
+    }
+}
+
+/* One chunk of a rotated table from the staging layout [q][X][entry][row]
+ * to [q][row][X][entry] (cs entries), through a 64-entry x TR-row LDS tile
+ * per block: loads run along rows, stores along entries. */
+template <typename E, int TR>
+__global__ void __launch_bounds__(256)
+k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr)
+{
+    __shared__ E tile[64][TR + 1];
+    int bid = blockIdx.x;
+    const int tr = bid % ntr;
+    bid /= ntr;
+    const int te = bid % nte;
+    const int qx = bid / nte;   /* q * 3 + X */
+    const int X = qx % 3, q = qx / 3;
+    const int e0 = te * 64, r0 = tr * TR;
+    const E *src = tmp + (long)qx * cs * nrows;
+    for (int k = threadIdx.x; k < 64 * TR; k += 256) {
+        const int i = k / TR, j = k - (k / TR) * TR;
+        const int e = e0 + i, r = r0 + j;
+        if (e < cs && r < nrows) tile[i][j] = src[(long)e * nrows + r];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 64 * TR; k += 256) {
+        const int j = k >> 6, i = k & 63;
+        const int e = e0 + i, r = r0 + j;
+        if (e < cs && r < nrows) out[(((long)q * nrows + r) * 3 + X) * cs + e] = tile[i][j];
     }
 }
 
@@ -768,7 +806,7 @@ enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
     B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
-    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_NBUF
+    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_RTMP, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -859,12 +897,27 @@ static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
 }
 
 template <typename E>
-static void launch_rotate(hipStream_t s, void *out, const cplx *srct, long nrot, size_t lds,
-                          int nb, int ng, int Q, int L, int gc)
+static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct, long nrot,
+                          size_t lds, int nb, int ng, int Q, int L, int gc, int C,
+                          const int *csz, const long *coff)
 {
-    hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)out, srct,
+    hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)tmp, srct,
                        (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
                        (const long *)g_buf[B_COFF], nb, ng, Q, L, gc);
+    /* 64-row tiles of FP32 tables, 32-row tiles of FP64 tables (33 KB of LDS) */
+    const int TR = sizeof(E) == 8 ? 64 : 32;
+    const int nrows = nb * ng;
+    for (int c = 0; c < C; c++) {
+        if (csz[c] == 0) continue;
+        const int nte = (csz[c] + 63) / 64, ntr = (nrows + TR - 1) / TR;
+        const long nblkT = (long)Q * 3 * nte * ntr;
+        if (sizeof(E) == 8)
+            hipLaunchKernelGGL((k_tab_transpose<E, 64>), dim3((unsigned)nblkT), dim3(256), 0, s,
+                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr);
+        else
+            hipLaunchKernelGGL((k_tab_transpose<E, 32>), dim3((unsigned)nblkT), dim3(256), 0, s,
+                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr);
+    }
 }
 
 template <typename E, int G>
@@ -1053,6 +1106,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     RCF(ensure(B_T, es * zn * Q * (size_t)Tn));
     RCF(ensure(B_RT, es * (size_t)nrows * spf));
     RCF(ensure(B_BT, es * (size_t)nrows * spf));
+    RCF(ensure(B_RTMP, es * (size_t)nrows * spf));
     RCF(ensure(B_A, sizeof(double) * 2 * spf));
     RCF(ensure(B_B, sizeof(double) * 2 * spf));
     RCF(ensure(B_DW, sizeof(double) * dsz * nb));
@@ -1141,18 +1195,26 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         /* gamma chunk: the staged rows (2L+1) x gc complex stay within 48 KB */
         int gc = ng < FSX_WAVE ? ng : FSX_WAVE;
         while (gc > 1 && (size_t)N * gc * sizeof(cplx) > 48 * 1024) gc /= 2;
-        const long nrot = (long)Q * 3 * K * ((ng + gc - 1) / gc);
-        const size_t lds_rot = (size_t)N * gc * sizeof(cplx);
+        const long nrot = (long)Q * 3 * K * ((nb + FSX_ROT_BC - 1) / FSX_ROT_BC) * ((ng + gc - 1) / gc);
+        const size_t lds_rot = (size_t)N * gc * sizeof(cplx) + (size_t)FSX_ROT_BC * N * N * sizeof(double);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dA, (const double *)(dA + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
-        if (mixed) launch_rotate<cplx32>(s, g_buf[B_RT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
-        else launch_rotate<cplx>(s, g_buf[B_RT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
+        if (mixed)
+            launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
+                                  L, gc, C, csz, coff);
+        else
+            launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
+                                gc, C, csz, coff);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
-        if (mixed) launch_rotate<cplx32>(s, g_buf[B_BT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
-        else launch_rotate<cplx>(s, g_buf[B_BT], srct, nrot, lds_rot, nb, ng, Q, L, gc);
+        if (mixed)
+            launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
+                                  L, gc, C, csz, coff);
+        else
+            launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
+                                gc, C, csz, coff);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
