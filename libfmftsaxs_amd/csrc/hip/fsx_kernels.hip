@@ -169,6 +169,50 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+/* Cross-lane steps of the 64-lane transpose-reduce, on the two 32-bit halves
+ * of a double.  v_permlane32_swap exchanges lanes 32..63 of its first operand
+ * with lanes 0..31 of its second, v_permlane16_swap the odd 16-lane rows of
+ * the first with the even rows of the second, so that the sum of the two
+ * results is, in every lane, its own slot's value plus the partner lane's
+ * value of the same slot (lanes below the split keep slot a, lanes above it
+ * slot b). */
+__device__ __forceinline__ double tr_swap32(double a, double b)
+{
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+    const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32),
+                                                     false, false);
+    const double r0 = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+    const double r1 = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+    return r0 + r1;
+}
+__device__ __forceinline__ double tr_swap16(double a, double b)
+{
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+    const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32),
+                                                     false, false);
+    const double r0 = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+    const double r1 = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+    return r0 + r1;
+}
+/* a double moved between lanes by a DPP pattern (row_ror:8 = lane xor 8,
+ * row_half_mirror = lane 7 - i in each group of 8, quad_perm) */
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+#define FSX_DPP_ROR8 0x128
+#define FSX_DPP_HMIRROR 0x141
+#define FSX_DPP_XOR1 0xB1
+#define FSX_DPP_XOR2 0x4E
+
 /* x mod n in [0, n) for |x| < 2^24, n >= 1: a float quotient estimate
  * (off by at most one) and one correction. */
 __device__ __forceinline__ int mod_small(int x, int n, float inv_n)
@@ -608,32 +652,20 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                 /* the cross-lane sums run in FP64 */
                 double v[8] = {va[0], va[1], va[2], va[3], va[4], va[5], 0.0, 0.0};
                 /* transpose-reduce over the 64 lanes: after the three halving
-                 * steps lane bits 5,4,3 select the slot, then a plain sum over
-                 * bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
+                 * steps lane bits 5,4,3 select the slot, then a sum over the
+                 * 8 lanes of bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
                 {
-                    bool hi = (lane & 32) != 0;
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        double send = hi ? v[i] : v[i + 4];
-                        double keep = hi ? v[i + 4] : v[i];
-                        v[i] = keep + __shfl_xor(send, 32, FSX_WAVE);
-                    }
-                    hi = (lane & 16) != 0;
+                    for (int i = 0; i < 4; i++) v[i] = tr_swap32(v[i], v[i + 4]);
 #pragma unroll
-                    for (int i = 0; i < 2; i++) {
-                        double send = hi ? v[i] : v[i + 2];
-                        double keep = hi ? v[i + 2] : v[i];
-                        v[i] = keep + __shfl_xor(send, 16, FSX_WAVE);
-                    }
-                    hi = (lane & 8) != 0;
-                    {
-                        double send = hi ? v[0] : v[1];
-                        double keep = hi ? v[1] : v[0];
-                        v[0] = keep + __shfl_xor(send, 8, FSX_WAVE);
-                    }
-                    v[0] += __shfl_xor(v[0], 4, FSX_WAVE);
-                    v[0] += __shfl_xor(v[0], 2, FSX_WAVE);
-                    v[0] += __shfl_xor(v[0], 1, FSX_WAVE);
+                    for (int i = 0; i < 2; i++) v[i] = tr_swap16(v[i], v[i + 2]);
+                    const bool hi = (lane & 8) != 0;
+                    const double send = hi ? v[0] : v[1];
+                    const double keep = hi ? v[1] : v[0];
+                    v[0] = keep + dpp_f64<FSX_DPP_ROR8>(send);
+                    v[0] += dpp_f64<FSX_DPP_HMIRROR>(v[0]);
+                    v[0] += dpp_f64<FSX_DPP_XOR1>(v[0]);
+                    v[0] += dpp_f64<FSX_DPP_XOR2>(v[0]);
                 }
                 if (writer) {
                     double x = 2.0 * v[0];
