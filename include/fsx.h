@@ -85,6 +85,9 @@ typedef struct fsx_stats {
     double cross_flops;  /* useful FP64 flops of the cross-term launches */
     long cross_items;    /* non-empty (z, beta2, gamma2) buckets x q */
     int cross_chunks;    /* m-chunks (cross-term launches) used */
+    int cross_entries;   /* coefficient entries per table row: (L+1)^2, or
+                            (L+1)(L+2)/2 when both molecules' coefficients are
+                            conjugate symmetric and only m >= 0 is formed */
 } fsx_stats;
 
 /* Score n poses.  score/c1/c2 receive one value per pose whose z slot is

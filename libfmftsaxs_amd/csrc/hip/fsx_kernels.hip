@@ -266,7 +266,7 @@ __global__ void k_twiddle(cplx *dst, const double *src_re, const double *src_im,
 template <typename E>
 __global__ void __launch_bounds__(256)
 k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
-         int nbeta, int ng, int Q, int L, int gc)
+         int nbeta, int ng, int Q, int L, int gc, int mlo)
 {
 #pragma clang fp contract(fast)
     extern __shared__ cplx sR[];   /* [2l+1][gc], then FSX_ROT_BC blocks [2l+1][2l+1] of d */
@@ -285,7 +285,8 @@ k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long 
     const int q = r / 3;
     const int g0 = ch * gc, gn = (ng - g0) < gc ? (ng - g0) : gc;
     const int b0 = bc * FSX_ROT_BC, nbh = nbeta - b0 < FSX_ROT_BC ? nbeta - b0 : FSX_ROT_BC;
-    const int w = 2 * l + 1, ww = w * w;
+    /* output rows m = mb..l (all m, or m >= 0 for the half entry set) */
+    const int w = 2 * l + 1, mb = mlo > -l ? mlo : -l, nm = l - mb + 1, ww = nm * w;
     double *sD = reinterpret_cast<double *>(sR + w * gc);
     const cplx *src = srct + (((long)q * 3 + X) * K2 + l * (l + 1) - l) * ng + g0;
     for (int i = threadIdx.x; i < w * gc; i += blockDim.x) {
@@ -294,19 +295,19 @@ k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long 
     }
     for (int i = threadIdx.x; i < nbh * ww; i += blockDim.x) {
         const int bb = i / ww, rem = i - bb * ww, mr = rem / w, k = rem - mr * w;
-        sD[i] = dW[(long)(b0 + bb) * K * N * N + ((long)l * N + (L - l + mr)) * N + (L - l + k)];
+        sD[i] = dW[(long)(b0 + bb) * K * N * N + ((long)l * N + (L + mb + mr)) * N + (L - l + k)];
     }
     __syncthreads();
-    const int np = (w + 1) / 2;   /* pairs of m */
+    const int np = (nm + 1) / 2;   /* pairs of m */
     for (int it = wave; it < nbh * np; it += nwave) {
         const int bb = it / np, mi = 2 * (it - bb * np), b = b0 + bb;
-        const bool two = mi + 1 < w;
+        const bool two = mi + 1 < nm;
         const double *d0 = sD + bb * ww + mi * w, *d1 = two ? d0 + w : d0;
         E *o[2];
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             /* entry (l, m) in m-major order */
-            const int m = (two ? mi + u : mi) - l;
+            const int m = mb + (two ? mi + u : mi);
             const int e = mstart[m + L] + (l - (m < 0 ? -m : m));
             const int c = ent_c[e];
             const long cs = csize[c];
@@ -931,11 +932,11 @@ static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
 template <typename E>
 static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct, long nrot,
                           size_t lds, int nb, int ng, int Q, int L, int gc, int C,
-                          const int *csz, const long *coff)
+                          const int *csz, const long *coff, int mlo)
 {
     hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)tmp, srct,
                        (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
-                       (const long *)g_buf[B_COFF], nb, ng, Q, L, gc);
+                       (const long *)g_buf[B_COFF], nb, ng, Q, L, gc, mlo);
     /* 64-row tiles of FP32 tables, 32-row tiles of FP64 tables (33 KB of LDS) */
     const int TR = sizeof(E) == 8 ? 64 : 32;
     const int nrows = nb * ng;
@@ -966,9 +967,11 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
                      int, int, int, int, int);
         if (tst)
-            kern = jn <= 1 ? k_cross<1, 1, G, E> : jn <= 2 ? k_cross<2, 1, G, E> : k_cross<4, 1, G, E>;
+            kern = jn <= 1 ? k_cross<1, 1, G, E> : jn <= 2 ? k_cross<2, 1, G, E>
+                 : jn <= 3 ? k_cross<3, 1, G, E> : k_cross<4, 1, G, E>;
         else
-            kern = jn <= 1 ? k_cross<1, 0, G, E> : jn <= 2 ? k_cross<2, 0, G, E> : k_cross<4, 0, G, E>;
+            kern = jn <= 1 ? k_cross<1, 0, G, E> : jn <= 2 ? k_cross<2, 0, G, E>
+                 : jn <= 3 ? k_cross<3, 0, G, E> : k_cross<4, 0, G, E>;
         size_t lds = sizeof(E) * (6 * (size_t)G * csz[c] + (tst ? Tn : 0));
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
@@ -1003,21 +1006,24 @@ static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, c
 }
 
 /* Entry tables of the m-major coefficient order used by the rotated tables
- * and the cross-term kernel: e runs over m = -L..L, then l = |m|..L.  The
- * entries are cut into C chunks of whole m values with about K2/C entries
- * each.  ent layout (ints): [0,K2) l, [K2,2K2) m, [2K2,3K2) chunk,
+ * and the cross-term kernel: e runs over m = mlo..L, then l = |m|..L (mlo =
+ * -L for the full set, 0 for the half set of conjugate-symmetric inputs).
+ * The entries are cut into C chunks of whole m values with about ne/C
+ * entries each.  ent layout (ints): [0,K2) l, [K2,2K2) m, [2K2,3K2) chunk,
  * [3K2,4K2) index within the chunk, [4K2, 4K2+C) chunk sizes,
  * [5K2, 5K2+2L+1) first entry of each m, [6K2, 6K2+C) chunk starts. */
-static void entry_tables(int L, int C, int *ent)
+static int entry_tables(int L, int C, int mlo, int *ent)
 {
-    const int K2 = (L + 1) * (L + 1);
+    const int K = L + 1, K2 = K * K;
+    const int ne = mlo == 0 ? K * (K + 1) / 2 : K2;
     int *el = ent, *em = ent + K2, *ec = ent + 2 * K2, *ei = ent + 3 * K2;
     int *csz = ent + 4 * K2, *mst = ent + 5 * K2, *cst = ent + 6 * K2;
     int e = 0, c = 0, cfirst = 0;
-    for (int m = -L; m <= L; m++) {
+    for (int m = -L; m <= L; m++) mst[m + L] = 0;
+    for (int m = mlo; m <= L; m++) {
         int am = m < 0 ? -m : m;
         /* close the chunk once it holds its share of the entries */
-        if (e > cfirst && c < C - 1 && (long)e * C >= (long)(c + 1) * K2) {
+        if (e > cfirst && c < C - 1 && (long)e * C >= (long)(c + 1) * ne) {
             csz[c] = e - cfirst;
             cst[c] = cfirst;
             c++;
@@ -1034,6 +1040,34 @@ static void entry_tables(int L, int C, int *ent)
     csz[c] = e - cfirst;
     cst[c] = cfirst;
     for (int k = c + 1; k < C; k++) { csz[k] = 0; cst[k] = e; }
+    return ne;
+}
+
+/* A^X_{l,-m}(q) = (-1)^(l+m) conj(A^X_{lm}(q)) for m >= 1, to 1e-13 of the
+ * largest coefficient of each (X, q): the amplitude of a real density.
+ * Then the terms of m and -m of every cross-term sum are complex
+ * conjugates of each other, and the sums run over m >= 0 with the m > 0
+ * terms counted twice. */
+static int spf_conj_symmetric(const double *re, const double *im, int Q, int L)
+{
+    const int K2 = (L + 1) * (L + 1);
+    for (long xq = 0; xq < 3L * Q; xq++) {
+        const double *r = re + xq * K2, *i = im + xq * K2;
+        double mx = 0.0;
+        for (int e = 0; e < K2; e++) {
+            const double a = fabs(r[e]), b = fabs(i[e]);
+            mx = a > mx ? a : mx;
+            mx = b > mx ? b : mx;
+        }
+        const double tol = 1e-13 * mx;
+        for (int l = 1; l <= L; l++)
+            for (int m = 1; m <= l; m++) {
+                const int ep = l * l + l + m, en = l * l + l - m;
+                const double sg = ((l + m) & 1) ? -1.0 : 1.0;
+                if (!(fabs(r[en] - sg * r[ep]) <= tol) || !(fabs(i[en] + sg * i[ep]) <= tol)) return 0;
+            }
+    }
+    return 1;
 }
 
 static int run_locked(const fsx_hip_args *A, fsx_stats *st)
@@ -1048,6 +1082,12 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     const size_t dsz = (size_t)K * N * N;
     const size_t spf = (size_t)3 * Q * K2;
     const int mixed = (A->flags & FSX_MIXED_FP32) != 0;
+    /* half entry set (m >= 0) when both molecules' coefficients are
+     * conjugate symmetric; FSX_CROSS_HALF=0 forces the full set */
+    const int half = env_int("FSX_CROSS_HALF", 1, 0, 1) &&
+                     spf_conj_symmetric(A->A_re, A->A_im, Q, L) &&
+                     spf_conj_symmetric(A->B_re, A->B_im, Q, L);
+    const int mlo = half ? 0 : -L;
     const size_t es = mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
     hipStream_t s = g_stream;
     /* m-chunks: one launch per chunk of entries; one chunk unless a chunk
@@ -1063,9 +1103,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         return FSX_ERR_NOMEM;
     }
     /* at most 4 entries per lane: more chunks until the largest fits */
-    int ncmax;
+    int ncmax, ne;
     for (;;) {
-        entry_tables(L, C, ent);
+        ne = entry_tables(L, C, mlo, ent);
         ncmax = 0;
         for (int c = 0; c < C; c++)
             if (ent[4 * K2 + c] > ncmax) ncmax = ent[4 * K2 + c];
@@ -1111,8 +1151,11 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         for (int m = -L; m <= L; m++) {
             long k = ((long)m * a2) % na;
             if (k < 0) k += na;
-            tw2[2 * ((long)a2 * N + m + L)] = A->tw_a[2 * k];
-            tw2[2 * ((long)a2 * N + m + L) + 1] = A->tw_a[2 * k + 1];
+            /* half set: the m > 0 terms stand for m and -m (an exact
+             * factor 2) */
+            const double f = half && m > 0 ? 2.0 : 1.0;
+            tw2[2 * ((long)a2 * N + m + L)] = f * A->tw_a[2 * k];
+            tw2[2 * ((long)a2 * N + m + L) + 1] = f * A->tw_a[2 * k + 1];
         }
     const int *csz = ent + 4 * K2, *cst = ent + 6 * K2;
     coff[0] = 0;
@@ -1136,9 +1179,10 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     {
     RCF(ensure(B_BES, sizeof(double) * zn * Q * N));
     RCF(ensure(B_T, es * zn * Q * (size_t)Tn));
-    RCF(ensure(B_RT, es * (size_t)nrows * spf));
-    RCF(ensure(B_BT, es * (size_t)nrows * spf));
-    RCF(ensure(B_RTMP, es * (size_t)nrows * spf));
+    const size_t spfe = (size_t)3 * Q * ne;   /* rotated-table entries per row */
+    RCF(ensure(B_RT, es * (size_t)nrows * spfe));
+    RCF(ensure(B_BT, es * (size_t)nrows * spfe));
+    RCF(ensure(B_RTMP, es * (size_t)nrows * spfe));
     RCF(ensure(B_A, sizeof(double) * 2 * spf));
     RCF(ensure(B_B, sizeof(double) * 2 * spf));
     RCF(ensure(B_DW, sizeof(double) * dsz * nb));
@@ -1234,19 +1278,19 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff);
+                                  L, gc, C, csz, coff, mlo);
         else
             launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff);
+                                gc, C, csz, coff, mlo);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff);
+                                  L, gc, C, csz, coff, mlo);
         else
             launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff);
+                                gc, C, csz, coff, mlo);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
@@ -1318,13 +1362,14 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
      * multiply-add of the translated ligand (3 parts, sum_e (L+1-|m_e|)
      * terms per item) and 54 per (pose, q, entry) of the pairing */
     double tpair = 0.0;
-    for (int e = 0; e < K2; e++) tpair += K - (ent[K2 + e] < 0 ? -ent[K2 + e] : ent[K2 + e]);
-    st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * K2 * 54.0;
+    for (int e = 0; e < ne; e++) tpair += K - (ent[K2 + e] < 0 ? -ent[K2 + e] : ent[K2 + e]);
+    st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * ne * 54.0;
     /* algorithmic bytes: per pose and q one receptor row (3 parts) and the
      * six-term accumulator (read and written by every chunk after the
      * first); per item one ligand row */
-    st->cross_bytes = (double)nvalid * Q * (3.0 * K2 * 16.0 + (2.0 * C - 1.0) * 48.0)
-                    + (double)items * 3.0 * K2 * 16.0;
+    st->cross_bytes = (double)nvalid * Q * (3.0 * ne * (double)es + (2.0 * C - 1.0) * 48.0)
+                    + (double)items * 3.0 * ne * (double)es;
+    st->cross_entries = ne;
     }
 out:
     free(ent);

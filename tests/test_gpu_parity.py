@@ -108,11 +108,25 @@ def test_extended_grid_vs_oracle(F):
     _cmp_results(pb, r_g, r_o)
 
 
+def _break_symmetry(pb, L, seed):
+    """The problem with A_{l,-m} (m < 0) scaled by 1 + 1e-3 noise, so that
+    the receptor coefficients are no longer conjugate symmetric."""
+    K2 = (L + 1) ** 2
+    rng = np.random.default_rng(seed)
+    Ar = np.array(pb["A_re"], dtype=np.float64).reshape(-1, K2).copy()
+    Ai = np.array(pb["A_im"], dtype=np.float64).reshape(-1, K2).copy()
+    neg = [l * l + l + m for l in range(1, L + 1) for m in range(-l, 0)]
+    Ar[:, neg] *= 1.0 + 1e-3 * rng.standard_normal((Ar.shape[0], len(neg)))
+    Ai[:, neg] *= 1.0 + 1e-3 * rng.standard_normal((Ai.shape[0], len(neg)))
+    return dict(pb, A_re=Ar.reshape(np.shape(pb["A_re"])), A_im=Ai.reshape(np.shape(pb["A_im"])))
+
+
 def test_large_L_chunked_vs_oracle(F):
-    """L = 25, the largest L of the workloads: 676 coefficients per row, so
-    the cross-term kernel runs as three m-chunk launches that accumulate in
-    place, and the translation blocks (6201 entries) are too large for LDS
-    and are read from global memory.  Both forms, at their bars."""
+    """L = 25, the largest L of the workloads.  The translation blocks (6201
+    entries) are too large for LDS and are read from global memory.  Physical
+    coefficients use the half entry set (351 per row, two m-chunk launches
+    that accumulate in place; both forms at their bars); coefficients without
+    the conjugate symmetry use all 676 entries (three chunks)."""
     L = 25
     zv = np.arange(30.0, 34.0)
     pb = build_problem(500, 300, L, 0.6, 11, zv, 24, seed=25)
@@ -120,17 +134,49 @@ def test_large_L_chunked_vs_oracle(F):
                                 pb["params"], pb["qvals"], zv, L, want_cross=True)
     gp = _gpu_problem(F, pb)
     r_g = F.score(gp, pb["index"], want_cross=True)
-    assert r_g["stats"]["cross_chunks"] == 3
+    st = r_g["stats"]
+    assert st["cross_entries"] in (351, 676)
+    assert st["cross_chunks"] == -(-st["cross_entries"] // 256)
     rel = _cmp_results(pb, r_g, r_o)
     r_m = F.score(gp, pb["index"], want_cross=True, mixed=True)
-    assert r_m["stats"]["cross_chunks"] == 3
+    assert r_m["stats"]["cross_chunks"] == st["cross_chunks"]
     ok = ~np.isnan(r_o["score"])
     relI = np.abs(_I(r_m["cross"][ok]) - _I(r_o["cross"][ok])) / np.abs(_I(r_o["cross"][ok]))
     relc = np.abs(r_m["score"][ok] - r_o["score"][ok]) / r_o["score"][ok]
-    print(f"L=25: FP64 max rel I {rel:.2e}; mixed max rel I {relI.max():.2e}, chi {relc.max():.2e}")
+    print(f"L=25 ({st['cross_entries']} entries): FP64 max rel I {rel:.2e}; "
+          f"mixed max rel I {relI.max():.2e}, chi {relc.max():.2e}")
     assert relI.max() <= 1e-5 and relc.max() <= 1e-5
     assert np.array_equal(np.argsort(r_m["score"][ok], kind="stable"),
                           np.argsort(r_o["score"][ok], kind="stable"))
+    pb2 = _break_symmetry(pb, L, 7)
+    r_o2 = O.compute_saxs_scores(pb2["index"], pb2["A_re"], pb2["A_im"], pb2["B_re"],
+                                 pb2["B_im"], pb2["params"], pb2["qvals"], zv, L, want_cross=True)
+    r_g2 = F.score(_gpu_problem(F, pb2), pb2["index"], want_cross=True)
+    assert r_g2["stats"]["cross_entries"] == 676 and r_g2["stats"]["cross_chunks"] == 3
+    _cmp_results(pb2, r_g2, r_o2)
+
+
+def test_entry_sets_symmetric_and_asymmetric(F):
+    """Coefficients of real densities satisfy A_{l,-m} = (-1)^(l+m)
+    conj(A_lm), so the kernels form only m >= 0 and count the m > 0 terms
+    twice; coefficients that break the symmetry take the full m = -L..L
+    entry set.  Both against the oracle, which always sums over all m."""
+    L = 8
+    zv = np.arange(20.0, 26.0)
+    pb = build_problem(300, 200, L, 0.3, 21, zv, 60, seed=12)
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L, want_cross=True)
+    r_g = F.score(_gpu_problem(F, pb), pb["index"], want_cross=True)
+    assert r_g["stats"]["cross_entries"] == (L + 1) * (L + 2) // 2
+    _cmp_results(pb, r_g, r_o)
+    K2 = (L + 1) ** 2
+    pb2 = _break_symmetry(pb, L, 3)
+    r_o2 = O.compute_saxs_scores(pb2["index"], pb2["A_re"], pb2["A_im"], pb2["B_re"],
+                                 pb2["B_im"], pb2["params"], pb2["qvals"], zv, L, want_cross=True)
+    r_g2 = F.score(_gpu_problem(F, pb2), pb2["index"], want_cross=True)
+    assert r_g2["stats"]["cross_entries"] == K2
+    _cmp_results(pb2, r_g2, r_o2)
+    assert not np.allclose(r_g2["cross"], r_g["cross"], rtol=1e-9, atol=0)
 
 
 def test_out_of_range_poses_untouched(F):
