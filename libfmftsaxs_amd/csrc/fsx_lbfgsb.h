@@ -65,16 +65,23 @@
 /* Cross-term profile of one pose: value of term k (0..5 = VV,VD,VW,DD,DW,WW)
  * at q index i is scale * (c[k*cks + i*cqs] + v[k*vks + i*vqs]); v may be
  * NULL.  This lets the GPU read (constant part + pose part) in place and
- * apply the peak rescale on the fly without a private copy. */
+ * apply the peak rescale on the fly without a private copy.  With c NULL,
+ * v holds the final values (already summed and rescaled).  gc, when set, is
+ * a per-fit cache of the factor G(q) = c1^3 exp(-mult (c1^2 - 1) q^2) at
+ * gc[i*gcs]: the scale pass stores it, the objective pass of the same point
+ * reads it back instead of evaluating the same expression again. */
 typedef struct fsx_ct {
     const double *c;
     const double *v;
     long cks, cqs, vks, vqs;
     double scale;
+    double *gc;
+    long gcs;
 } fsx_ct;
 
 FSX_HD double fsx_ct_get(const fsx_ct *t, int k, int i)
 {
+    if (!t->c) return t->v[k * t->vks + i * t->vqs];
     double x = t->c[k * t->cks + i * t->cqs];
     if (t->v) x = x + t->v[k * t->vks + i * t->vqs];
     return x * t->scale;
@@ -94,6 +101,11 @@ typedef struct fsx_fitpar {
  * all its loads of a step in flight at once. */
 FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 {
+    if (!t->c) {
+        const double *v = t->v + i * t->vqs;
+        for (int k = 0; k < 6; k++) o[k] = v[k * t->vks];
+        return;
+    }
     const double *c = t->c + i * t->cqs;
     if (t->v) {
         const double *v = t->v + i * t->vqs;
@@ -112,6 +124,27 @@ FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 #define FSX_QB 4
 FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
 {
+    if (!t->c) {
+        const double *v = t->v + i0 * t->vqs;
+        const long vks = t->vks, vqs = t->vqs;
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (vks == 1 && vqs == 6) {
+            const double2 *v2 = reinterpret_cast<const double2 *>(v);
+#pragma unroll
+            for (int u = 0; u < FSX_QB; u++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double2 p = u < nb ? v2[u * 3 + k] : make_double2(0.0, 0.0);
+                    blk[u][2 * k] = p.x;
+                    blk[u][2 * k + 1] = p.y;
+                }
+            return;
+        }
+#endif
+        for (int u = 0; u < FSX_QB; u++)
+            for (int k = 0; k < 6; k++) blk[u][k] = u < nb ? v[u * vqs + k * vks] : 0.0;
+        return;
+    }
     double cr[FSX_QB][6];
     const double *c = t->c + i0 * t->cqs;
     const long cks = t->cks, cqs = t->cqs;
@@ -203,6 +236,7 @@ FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, do
                 const double *cur = blk[u];
                 double qc = q[i];
                 G = c1c * exp(corr * qc * qc);
+                if (t->gc) t->gc[i * t->gcs] = G;
                 double in = FSX_IN(cur, G, c2);
                 double tan_ = (in - in_prev) / (qc - q_prev);
                 double buf = in - tan_ * qc;
@@ -228,7 +262,7 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
     double c1 = x[0], c2 = x[1];
     double g0 = 0.0, g1 = 0.0, mult = p->mult;
     double corr = -mult * (c1 * c1 - 1.0);
-    double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
+    double G = t->gc ? t->gc[0] : c1 * c1 * c1 * exp(corr * q[0] * q[0]);
     double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * q[0] * q[0]);
     double in_prev = FSX_IN(v0, G, c2);
     double d1p = -Gd * v0[1] + 2.0 * G * Gd * v0[3] - Gd * c2 * v0[4];
@@ -245,7 +279,7 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
             if (i < qnum) {
                 const double *cur = blk[u];
                 double qc = q[i];
-                G = c1c * exp(corr * qc * qc);
+                G = t->gc ? t->gc[i * t->gcs] : c1c * exp(corr * qc * qc);
                 Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
                 double vd = cur[1], vw = cur[2], dd = cur[3];
                 double dw = cur[4], ww = cur[5];

@@ -530,8 +530,12 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
         for (int z = 0; z < zn; z++) {
             /* the tile's buckets are adjacent keys, their poses adjacent in recs */
             const int key = (z * nbeta + b2) * ng + g20;
-            const int cnt0 = counts[key];
-            const int cnt = G == 2 && nr == 2 ? cnt0 + counts[key + 1] : cnt0;
+            /* first pose of each row of the tile (rows past nr: none) */
+            int cum[G + 1];
+            cum[0] = 0;
+#pragma unroll
+            for (int r = 0; r < G; r++) cum[r + 1] = cum[r] + (r < nr ? counts[key + r] : 0);
+            const int cnt = cum[G];
             if (cnt == 0) continue;
             const int start = offs[key];
             const E *Tq = T + ((long)z * Q + q) * Tn;
@@ -542,7 +546,9 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
             int s = 0, prow = 0;
             auto load_pose = [&](int p) {
                 s = start + p;
-                prow = p < cnt0 ? 0 : 1;
+                prow = 0;
+#pragma unroll
+                for (int r = 1; r < G; r++) prow += p >= cum[r];
                 int4 rec = recs[s];
                 const E *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
 #pragma unroll
@@ -624,7 +630,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
             __syncthreads();
             for (int p = wave; p < cnt; p += nwave) {
                 if (p != wave) load_pose(p);
-                const E *sUp = sU + (G == 2 ? prow * 3 * nc : 0);
+                const E *sUp = sU + prow * 3 * nc;
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
@@ -685,46 +691,79 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
  * whose fit ended stores the result and takes the next pose from the queue.
  * Profiles: pose-major acc[(s*Q + q)*6 + k] plus the constant parts cint[k*Q
  * + q], rescaled by peak / I(q0) (min_saxs.c:170-179). */
-__device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitpar *fp,
-                                          const double *acc, int s, int Q, double peak,
-                                          double *cross_out, int pose)
+/* Scale of each pose's profile, peak / I(q0) with I(q0) the six terms at
+ * q index 0 summed as min_saxs.c:170-179 (one thread per pose slot). */
+__global__ void k_fit_scale(const double *acc, const double *cint, const int *nvalid_p, int Q,
+                            double peak, double *scale)
 {
 #pragma clang fp contract(off)
-    t->v = acc + (long)s * Q * 6;
-    t->scale = 1.0;
-    double sc = fsx_ct_get(t, 0, 0) + fsx_ct_get(t, 3, 0) + fsx_ct_get(t, 5, 0)
-              + fsx_ct_get(t, 2, 0) - fsx_ct_get(t, 1, 0) - fsx_ct_get(t, 4, 0);
-    if (cross_out) {
-        for (int k = 0; k < 6; k++)
-            for (int q = 0; q < Q; q++)
-                cross_out[((long)pose * 6 + k) * Q + q] = fsx_ct_get(t, k, q);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= *nvalid_p) return;
+    const double *v = acc + (long)s * Q * 6;
+    double g[6];
+    for (int k = 0; k < 6; k++) {
+        double x = cint[k * Q];
+        x = x + v[k];
+        g[k] = x * 1.0;
     }
-    t->scale = peak / sc;
+    const double sc = g[0] + g[3] + g[5] + g[2] - g[1] - g[4];
+    scale[s] = peak / sc;
+}
+
+/* The fit's profiles, (constant part + pose part) * scale, in place of the
+ * pose part, and the unscaled profiles to cross_out when asked; one thread
+ * per (pose slot, q, term). */
+__global__ void k_fit_prep(double *acc, const double *cint, const int4 *recs,
+                           const int *nvalid_p, int Q, const double *scale, double *cross_out)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n6 = (long)Q * 6;
+    if (i >= (long)*nvalid_p * n6) return;
+    const int s = (int)(i / n6), r = (int)(i - (long)s * n6), q = r / 6, k = r - q * 6;
+    double x = cint[k * Q + q];
+    x = x + acc[i];
+    if (cross_out) cross_out[((long)recs[s].x * 6 + k) * Q + q] = x * 1.0;
+    acc[i] = x * scale[s];
+}
+
+__device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitpar *fp,
+                                          const double *acc, int s, int Q)
+{
+    t->v = acc + (long)s * Q * 6;
     fsx_lb_start(S, fp, 1.0, 0.0);
 }
 
+/* the G(q) cache of the fit kernel: Q doubles per thread in LDS while they
+ * fit in FSX_FIT_GCACHE bytes per block of 256 */
+#define FSX_FIT_GCACHE (128 * 1024)
+
 __global__ void __launch_bounds__(256)
-k_fit(const double *acc, const double *cint, const int4 *recs, const int *nvalid_p, int Q,
-      const double *qvals, const double *a, double mult, double peak, double c1_lo,
-      double c1_hi, double c2_lo, double c2_hi, double *score, double *c1o, double *c2o,
-      double *cross_out, int *next, unsigned long long *tally)
+k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+      const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+      double *score, double *c1o, double *c2o, int *next, unsigned long long *tally,
+      int gcache)
 {
 #pragma clang fp contract(off)
+    extern __shared__ double sG[];   /* [Q][256] when gcache */
     const int nvalid = *nvalid_p;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     fsx_fitpar fp;
     fp.a = a; fp.q = qvals; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
+    /* profiles already rescaled in place (k_fit_prep) */
     fsx_ct t;
-    t.c = cint; t.cks = Q; t.cqs = 1;
+    t.c = NULL; t.cks = 0; t.cqs = 0;
     t.v = acc; t.vks = 1; t.vqs = 6;
     t.scale = 1.0;
+    t.gc = gcache ? sG + threadIdx.x : NULL;
+    t.gcs = blockDim.x;
     fsx_lbrc S;
     unsigned long long nev = 0, ncap = 0;
     int act = s < nvalid;
     int pose = act ? recs[s].x : 0;
-    if (act) fit_begin(&t, &S, &fp, acc, s, Q, peak, cross_out, pose);
+    if (act) fit_begin(&t, &S, &fp, acc, s, Q);
     while (__any(act)) {
         if (act) {
             double k;
@@ -740,7 +779,7 @@ k_fit(const double *acc, const double *cint, const int4 *recs, const int *nvalid
             act = s < nvalid;
             if (act) {
                 pose = recs[s].x;
-                fit_begin(&t, &S, &fp, acc, s, Q, peak, cross_out, pose);
+                fit_begin(&t, &S, &fp, acc, s, Q);
             }
         }
     }
@@ -759,6 +798,7 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     fsx_ct t;
     t.c = ct + (long)s * 6 * Q; t.cks = Q; t.cqs = 1;
     t.v = NULL; t.vks = 0; t.vqs = 0; t.scale = 1.0;
+    t.gc = NULL; t.gcs = 0;
     fsx_fitpar fp;
     fp.a = a; fp.q = qvals; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
@@ -839,7 +879,7 @@ enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
     B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
-    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_RTMP, B_NBUF
+    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_RTMP, B_FITSC, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -1000,8 +1040,9 @@ static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, c
                          int Tn, long grid, int nrows, int Q, int L, int nb, int na, int ng,
                          int zn)
 {
-    const int G = env_int("FSX_CROSS_ROWS", 2, 1, 2);
-    if (G == 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    const int G = env_int("FSX_CROSS_ROWS", 2, 1, 4);
+    if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
     else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
 }
 
@@ -1200,6 +1241,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     RCF(ensure(B_FILL, sizeof(int) * nkeys));
     RCF(ensure(B_RECS, sizeof(int4) * n));
     RCF(ensure(B_ACC, sizeof(double) * 6 * Q * (size_t)n));
+    RCF(ensure(B_FITSC, sizeof(double) * (size_t)(n > 0 ? n : 1)));
     RCF(ensure(B_ENT, sizeof(int) * nent));
     RCF(ensure(B_TW2, sizeof(double) * 2 * (size_t)na * N));
     RCF(ensure(B_COFF, sizeof(long) * (C + 1)));
@@ -1314,20 +1356,32 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue: as many lanes as are resident at once
      * (the fit state lives in registers, one 256-thread block per CU) */
+    const int gcache = env_int("FSX_FIT_GCACHE", 1, 0, 1) &&
+                       (size_t)Q * 256 * sizeof(double) <= FSX_FIT_GCACHE;
+    const size_t lds_fit = gcache ? (size_t)Q * 256 * sizeof(double) : 0;
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
-    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, 0)
+    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, lds_fit)
                              != hipSuccess || fit_bpc < 1))
         fit_bpc = 1;
     long lanes = (long)g_ncu * 256 * fit_bpc;
     if (lanes > n) lanes = ((long)n + 255) / 256 * 256;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_NEXT], 1L, (int)lanes);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_TALLY], 4L, 0);
-    hipLaunchKernelGGL(k_fit, dim3((unsigned)(lanes / 256)), dim3(256), 0, s,
+    /* rescaled profiles in place of the pose parts (bounded by n slots;
+     * the kernels read the number of valid slots on the device) */
+    hipLaunchKernelGGL(k_fit_scale, dim3(nblk(n, 256)), dim3(256), 0, s,
                        (const double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
+                       (const int *)(offs + nkeys), Q, A->peak, (double *)g_buf[B_FITSC]);
+    hipLaunchKernelGGL(k_fit_prep, dim3(nblk((long)n * Q * 6, 256)), dim3(256), 0, s,
+                       (double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
                        (const int4 *)g_buf[B_RECS], (const int *)(offs + nkeys), Q,
-                       (const double *)g_buf[B_Q], (const double *)g_buf[B_AM], A->mult, A->peak,
-                       A->c1_lo, A->c1_hi, A->c2_lo, A->c2_hi, dscore, dc1, dc2, dcross,
-                       (int *)g_buf[B_NEXT], (unsigned long long *)g_buf[B_TALLY]);
+                       (const double *)g_buf[B_FITSC], dcross);
+    hipLaunchKernelGGL(k_fit, dim3((unsigned)(lanes / 256)), dim3(256), lds_fit, s,
+                       (const double *)g_buf[B_ACC], (const int4 *)g_buf[B_RECS],
+                       (const int *)(offs + nkeys), Q, (const double *)g_buf[B_Q],
+                       (const double *)g_buf[B_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo,
+                       A->c2_hi, dscore, dc1, dc2, (int *)g_buf[B_NEXT],
+                       (unsigned long long *)g_buf[B_TALLY], gcache);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[5], s));
     if (!dev_out) {

@@ -348,6 +348,7 @@ static fsx_ct ct_packed(const double *c, int qnum)
 {
     fsx_ct t;
     t.c = c; t.v = NULL; t.cks = qnum; t.cqs = 1; t.vks = 0; t.vqs = 0; t.scale = 1.0;
+    t.gc = NULL; t.gcs = 0;
     return t;
 }
 
