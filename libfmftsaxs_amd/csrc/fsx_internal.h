@@ -33,6 +33,10 @@ typedef struct fsx_hip_args {
     int n;
     double *score, *c1, *c2, *cross;
     unsigned flags;
+    /* nonzero keys identify the content of dW / dsymb: a table whose key
+     * matches the one last uploaded into the same device buffer is not
+     * uploaded again */
+    unsigned long long dw_key, ds_key;
 } fsx_hip_args;
 
 int fsx_hip_run(const fsx_hip_args *args, fsx_stats *st);

@@ -883,6 +883,9 @@ enum {
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
+/* content key of what a buffer holds (fsx_hip_args dw_key / ds_key); 0 =
+ * unknown.  Reset whenever the buffer is (re)allocated or released. */
+static unsigned long long g_key[B_NBUF];
 
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
@@ -901,6 +904,7 @@ static int ensure(int slot, size_t bytes)
     if (g_buf[slot]) HIPCHK(hipFree(g_buf[slot]));
     g_buf[slot] = NULL;
     g_cap[slot] = 0;
+    g_key[slot] = 0;
     size_t cap = bytes + bytes / 8;
     if (hipMalloc(&g_buf[slot], cap) != hipSuccess) {
         g_buf[slot] = NULL;
@@ -949,6 +953,7 @@ extern "C" void fsx_hip_release(void)
         if (g_buf[i]) (void)hipFree(g_buf[i]);
         g_buf[i] = NULL;
         g_cap[i] = 0;
+        g_key[i] = 0;
     }
     pthread_mutex_unlock(&g_mtx);
 }
@@ -1253,8 +1258,14 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipMemcpyAsync(dA + spf, A->A_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(dB, A->B_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(dB + spf, A->B_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_DW], A->dW, sizeof(double) * dsz * nb, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
+    if (!(A->dw_key && g_key[B_DW] == A->dw_key)) {
+        HIPF(hipMemcpyAsync(g_buf[B_DW], A->dW, sizeof(double) * dsz * nb, hipMemcpyHostToDevice, s));
+        g_key[B_DW] = A->dw_key;
+    }
+    if (!(A->ds_key && g_key[B_DS] == A->ds_key)) {
+        HIPF(hipMemcpyAsync(g_buf[B_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
+        g_key[B_DS] = A->ds_key;
+    }
     HIPF(hipMemcpyAsync(g_buf[B_TWA], A->tw_a, sizeof(double) * 2 * na, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(g_buf[B_TWG], A->tw_g, sizeof(double) * 2 * ng, hipMemcpyHostToDevice, s));
     HIPF(hipMemcpyAsync(g_buf[B_Q], A->qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s));

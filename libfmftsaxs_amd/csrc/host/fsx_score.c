@@ -14,6 +14,7 @@
 #include "fftsaxs.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -67,10 +68,35 @@ static void const_terms(const fsx_problem *pb, double *cint)
         }
 }
 
+/* The Wigner-d tables of the last beta grid, kept across calls (they depend
+ * only on L, nbeta and pi); fsx_score calls are serialised on this lock. */
+static pthread_mutex_t g_dw_lock = PTHREAD_MUTEX_INITIALIZER;
+static struct {
+    int L, nbeta;
+    double pi;
+    double *dW;
+    unsigned long long key;
+} g_dw;
+static unsigned long long g_dw_serial;
+
+static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
+                            double *c1, double *c2, double *cross, unsigned flags,
+                            fsx_stats *st, double t0);
+
 int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
               double *c1, double *c2, double *cross, unsigned flags, fsx_stats *st)
 {
     double t0 = now_ms();
+    pthread_mutex_lock(&g_dw_lock);
+    int rc = fsx_score_locked(pb, poses, n, score, c1, c2, cross, flags, st, t0);
+    pthread_mutex_unlock(&g_dw_lock);
+    return rc;
+}
+
+static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
+                            double *c1, double *c2, double *cross, unsigned flags,
+                            fsx_stats *st, double t0)
+{
     fsx_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof(*st));
@@ -85,17 +111,31 @@ int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
     const double *ds = fsx_d_symb_cached(L);
     if (!ds) return FSX_ERR_NOMEM;
     double *cint = (double *)malloc(sizeof(double) * 6 * pb->qnum);
-    double *dW = (double *)malloc(sizeof(double) * dsz * pb->nbeta);
     double *twa = (double *)malloc(sizeof(double) * 2 * pb->nalpha);
     double *twg = (double *)malloc(sizeof(double) * 2 * pb->ngamma);
-    if (!cint || !dW || !twa || !twg) {
-        free(cint); free(dW); free(twa); free(twg);
+    if (!cint || !twa || !twg) {
+        free(cint); free(twa); free(twg);
         return FSX_ERR_NOMEM;
     }
     const_terms(pb, cint);
-    /* fftsaxs.c:632-636, 706-708: beta_i = 0 + i * (pi - 0)/(nbeta - 1) */
-    double bstep = (pb->pi - 0.0) / (pb->nbeta - 1);
-    for (int b = 0; b < pb->nbeta; b++) fsx_wigner_d(L, 0.0 + b * bstep, dW + b * dsz);
+    if (!(g_dw.dW && g_dw.L == L && g_dw.nbeta == pb->nbeta &&
+          memcmp(&g_dw.pi, &pb->pi, sizeof(double)) == 0)) {
+        double *dW = (double *)malloc(sizeof(double) * dsz * pb->nbeta);
+        if (!dW) {
+            free(cint); free(twa); free(twg);
+            return FSX_ERR_NOMEM;
+        }
+        /* fftsaxs.c:632-636, 706-708: beta_i = 0 + i * (pi - 0)/(nbeta - 1) */
+        double bstep = (pb->pi - 0.0) / (pb->nbeta - 1);
+        for (int b = 0; b < pb->nbeta; b++) fsx_wigner_d(L, 0.0 + b * bstep, dW + b * dsz);
+        free(g_dw.dW);
+        g_dw.dW = dW;
+        g_dw.L = L;
+        g_dw.nbeta = pb->nbeta;
+        g_dw.pi = pb->pi;
+        g_dw.key = ++g_dw_serial;
+    }
+    const double *dW = g_dw.dW;
     /* exact twiddles e^{-2 pi i k / N} (the FFTW transform of fftsaxs.c:680) */
     const double two_pi = 6.283185307179586476925286766559;
     for (int k = 0; k < pb->nalpha; k++) {
@@ -112,13 +152,15 @@ int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
     a.qnum = pb->qnum; a.znum = pb->znum; a.qvals = pb->qvals; a.zvals = pb->zvals;
     a.A_re = pb->A_re; a.A_im = pb->A_im; a.B_re = pb->B_re; a.B_im = pb->B_im;
     a.a = pb->a; a.cint = cint; a.dW = dW; a.dsymb = ds; a.tw_a = twa; a.tw_g = twg;
+    a.dw_key = g_dw.key;
+    a.ds_key = (unsigned long long)L + 1;   /* the d-symbol table of L never changes */
     a.mult = pb->mult; a.peak = pb->peak;
     a.c1_lo = pb->c1_lo; a.c1_hi = pb->c1_hi; a.c2_lo = pb->c2_lo; a.c2_hi = pb->c2_hi;
     a.poses = poses; a.n = n; a.score = score; a.c1 = c1; a.c2 = c2; a.cross = cross;
     a.flags = flags;
     st->ms_tables = now_ms() - t0;
     int rc = fsx_hip_run(&a, st);
-    free(cint); free(dW); free(twa); free(twg);
+    free(cint); free(twa); free(twg);
     st->ms_total = now_ms() - t0;
     return rc;
 }
