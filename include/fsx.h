@@ -122,6 +122,27 @@ int fsx_atoms2spf(int natoms, const double *coords, const double *vac, const dou
                   const double *sa, double h2o_ff, const double *vdw, const double *qvals,
                   int qnum, int L, double *re, double *im, double *rm);
 
+/* Brute-force per-pose scoring, the loop of tools/score_ft_naive.c:156-212
+ * as a library call.  xyz [nrec + nlig][3]: receptor atoms first, then ligand
+ * atoms, each molecule in its starting frame (the tool centres both).  Pose
+ * i = poses[6i .. 6i+5] = (z, b1, g1, a2, b2, g2): the receptor is rotated by
+ * R(0, b1, g1), the ligand by R(a2, b2, g2) and moved by (0, 0, z)
+ * (sxs_fill_active_rotation_matrix), the complex is centred at its centroid,
+ * expanded to L (fsx_atoms2spf, sa of the starting complex) and its profile
+ * fitted as sxs_spf2fitted_profile (cross terms, then L-BFGS-B from
+ * (c1, c2) = (1, 0) in the given box, no peak rescale).  a [6][qnum] and mult
+ * are those of sxs_opt_params for the experimental profile and the joint mean
+ * radius.  out [n][3]: score, c1, c2; cross (optional, may be NULL)
+ * [n][6][qnum]: VV, VD, VW, DD, DW, WW of the complex.  Independent of the
+ * FFT correlation: grid pose (b1, g1, a2, b2, g2) of fsx_score is the real-space
+ * pose (b1, g1, 2pi - a2, b2, 2pi - g2) here, the mirroring of the Euler-list
+ * reader, and the two agree to the truncation error at L. */
+int fsx_score_naive(int nrec, int nlig, const double *xyz, const double *vac,
+                    const double *dum, const double *sa, double h2o_ff, const double *vdw,
+                    const double *qvals, int qnum, int L, const double *a, double mult,
+                    double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                    const double *poses, int n, double *out, double *cross);
+
 /* Bessel table j_p(x_i) for p = 0..pmax on the GPU, x host [n]; out [n][pmax+1] */
 int fsx_sbessel_batch(const double *x, int n, int pmax, double *out);
 

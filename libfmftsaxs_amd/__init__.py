@@ -92,6 +92,10 @@ def lib():
         L.fsx_atoms2spf.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double, _dp, _dp,
                                     C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.fsx_atoms2spf.restype = C.c_int
+        L.fsx_score_naive.argtypes = ([C.c_int, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double,
+                                       _dp, _dp, C.c_int, C.c_int, _dp] + [C.c_double] * 5 +
+                                      [_dp, C.c_int, _dp, C.c_void_p])
+        L.fsx_score_naive.restype = C.c_int
         L.fsx_device_count.restype = C.c_int
         L.fsx_error_string.argtypes = [C.c_int]
         L.fsx_error_string.restype = C.c_char_p
@@ -236,6 +240,35 @@ def atoms2spf(coords, vac, dum, sa, vdw, qvals, L, h2o_ff=3.50968):
                              None if sa_arr is None else sa_arr.ctypes.data, h2o_ff, f64(vdw),
                              qvals, qvals.size, L, re.reshape(-1), im.reshape(-1), C.byref(rm)))
     return re, im, rm.value
+
+
+def score_naive(rec, lig, qvals, L, a, mult, poses, c1=(0.96, 1.04), c2=(-2.0, 4.0),
+                sa=None, h2o_ff=3.50968, want_cross=False):
+    """Brute-force per-pose scores (tools/score_ft_naive.c:156-212): rec/lig
+    are (coords [n,3], vac, dum, vdw) tuples in their starting frames, sa the
+    per-atom SASA fraction of the starting complex (receptor atoms first) or
+    None, poses [n,6] = (z, b1, g1, a2, b2, g2).  Returns a dict with score,
+    c1, c2 and, if asked, the cross-term profiles [n,6,Q]."""
+    xyz = f64(np.vstack([f64(rec[0]).reshape(-1, 3), f64(lig[0]).reshape(-1, 3)]))
+    nrec, nlig = f64(rec[0]).reshape(-1, 3).shape[0], f64(lig[0]).reshape(-1, 3).shape[0]
+    vac = f64(np.concatenate([rec[1], lig[1]]))
+    dum = f64(np.concatenate([rec[2], lig[2]]))
+    vdw = f64(np.concatenate([rec[3], lig[3]]))
+    poses = f64(poses).reshape(-1, 6)
+    qvals = f64(qvals)
+    n = poses.shape[0]
+    out = np.zeros((n, 3))
+    cross = np.zeros((n, 6, qvals.size)) if want_cross else None
+    sa_arr = None if sa is None else f64(sa)
+    _chk(lib().fsx_score_naive(nrec, nlig, xyz.ravel(), vac, dum,
+                               None if sa_arr is None else sa_arr.ctypes.data, h2o_ff, vdw,
+                               qvals, qvals.size, L, f64(a), mult, c1[0], c1[1], c2[0], c2[1],
+                               poses.ravel(), n, out.reshape(-1),
+                               None if cross is None else cross.ctypes.data))
+    r = {"score": out[:, 0].copy(), "c1": out[:, 1].copy(), "c2": out[:, 2].copy()}
+    if want_cross:
+        r["cross"] = cross
+    return r
 
 
 def sbessel(l, x):

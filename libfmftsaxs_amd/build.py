@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libfmftsaxs_amd.so")
 INCLUDE = os.path.join(ROOT, "include")
 
 HOST_SRC = ["host/fsx_math.c", "host/fsx_api.c", "host/fsx_score.c", "host/fsx_spf.c",
-            "host/fsx_front.c"]
+            "host/fsx_front.c", "host/fsx_naive.c"]
 HIP_SRC = ["hip/fsx_kernels.hip"]
 
 CFLAGS = ["-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",

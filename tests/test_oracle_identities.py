@@ -224,3 +224,65 @@ def test_index_round_trip_and_carry(orc):
     # a pose whose z is not within 0.001 of a grid z is dropped (tools/correlate.c:219-220)
     assert orc.euler2index(40.002, 1.0, 1.0, 1.0, 1.0, 1.0, zv, L) == []
     assert pi == 3.14159265358
+
+
+def _rot(a, b, g):
+    """active zyz rotation R(alpha, beta, gamma) in closed form (saxs_utils.c:65-79)"""
+    ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(g), np.sin(g)
+    return np.array([[cg * cb * ca - sg * sa, -sg * cb * ca - cg * sa, sb * ca],
+                     [cg * cb * sa + sg * ca, -sg * cb * sa + cg * ca, sb * sa],
+                     [-cg * sb, sg * sb, cb]])
+
+
+def _real_space_cross(orc, rec, lig, q, L, z, b1, g1, a2, b2, g2):
+    """tools/score_ft_naive.c:156-190: receptor R(0, b1, g1) x, ligand
+    R(a2, b2, g2) x + (0, 0, z), the complex centred at its centroid and
+    expanded directly, then its cross terms (min_saxs.c:450-505)."""
+    xr = rec.coords @ _rot(0.0, b1, g1).T
+    xl = lig.coords @ _rot(a2, b2, g2).T + np.array([0.0, 0.0, z])
+    x = np.vstack([xr, xl])
+    x -= x.mean(axis=0)
+    cat = lambda f: np.concatenate([getattr(rec, f), getattr(lig, f)])
+    re, im, _ = orc.atoms2spf(x, cat("vac"), cat("dum"), cat("sa"), cat("vdw"), q, L)
+    return orc.spf2cross_terms(re, im, L, q.size)
+
+
+def test_fft_pose_equals_real_space_complex(orc):
+    """The correlation's pose geometry against an independent real-space
+    construction of the same complex.  A grid pose (b1, g1, a2, b2, g2) is
+    the real-space pose with alpha2 and gamma2 mirrored to 2 pi - angle (the
+    mirroring of the Euler-list reader, correlate.c:222-223); the cross terms
+    agree to the truncation error at L, which falls with L down to the
+    accuracy of the reference's spherical Bessel routine (~6e-7 relative
+    against scipy, so ~3e-8 in I(q) here), and the unmirrored pose does not
+    agree."""
+    from libfmftsaxs_amd import synthetic as S
+    orc.set_pi(orc.PI_EXACT)
+    rec, lig = S.make_dimer_inputs(60, 40, 11, 12, water=True)
+    q = orc.mkarray(0.0, 0.15, 6)
+    z = 25.0
+    I = lambda c: c[0] - c[1] + c[2] + c[3] - c[4] + c[5]
+    err = {}
+    for L in (6, 30):
+        N = 2 * L + 1
+        Ar, Ai, _ = orc.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
+        Br, Bi, _ = orc.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
+        P = orc.Params(np.ones(6 * q.size), 1.6, 1.0)
+        for (ib1, ig1, ia2, ib2, ig2) in [(L // 4, 11, 23, 2 * L // 3, 41 % N), (3, N - 2, 5, L - 1, 9)]:
+            idx = np.array([orc.assemble_index(0, ib1, ib2, ia2, ig1, ig2, L + 1, N, N)],
+                           dtype=np.int64)
+            cf = orc.compute_saxs_scores(idx, Ar, Ai, Br, Bi, P, q, np.array([z]), L,
+                                         want_cross=True)["cross"][0]
+            b1, b2 = ib1 * np.pi / L, ib2 * np.pi / L
+            g1, a2, g2 = 2 * np.pi * ig1 / N, 2 * np.pi * ia2 / N, 2 * np.pi * ig2 / N
+            cm = _real_space_cross(orc, rec, lig, q, L, z, b1, g1, 2 * np.pi - a2, b2,
+                                   2 * np.pi - g2)
+            rel = np.abs(I(cm) - I(cf)).max() / np.abs(I(cf)).max()
+            err[L] = max(err.get(L, 0.0), rel)
+            if L == 30:
+                scale = np.abs(cf).max(axis=1, keepdims=True)
+                assert np.all(np.abs(cm - cf) <= 1e-6 * scale)
+                cu = _real_space_cross(orc, rec, lig, q, L, z, b1, g1, a2, b2, g2)
+                assert np.abs(I(cu) - I(cf)).max() / np.abs(I(cf)).max() > 1e-3
+    print("I(q) rel. error by L:", err)
+    assert err[30] < 1e-7 and err[6] > 100 * err[30], err
