@@ -201,12 +201,18 @@ __device__ __forceinline__ double tr_swap16(double a, double b)
 /* a double moved between lanes by a DPP pattern (row_ror:8 = lane xor 8,
  * row_half_mirror = lane 7 - i in each group of 8, quad_perm) */
 template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v)
+__device__ __forceinline__ double dpp_mov(double v)
 {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
     const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
     const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                                 0xf, 0xf, false));
 }
 #define FSX_DPP_ROR8 0x128
 #define FSX_DPP_HMIRROR 0x141
@@ -477,7 +483,7 @@ template <int J, int TST, int G, typename E>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 5 : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
-        const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int na, int ng,
+        const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
         int zn, int first)
 {
 #pragma clang fp contract(fast)
@@ -492,19 +498,33 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
     const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
     const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
-    (void)na;
-    /* per-thread U entry (entries ordered by decreasing row length, so that
-     * the lanes of a wavefront run loops of nearly equal length), per-lane
-     * pairing entries */
+    /* U items (entries ordered by decreasing row length K - |m|, so that
+     * the lanes of a wavefront run loops of nearly equal length).  The
+     * nsplit longest entries are split into the two halves of their l1
+     * range, run by lanes 2i and 2i + 1 and summed across the pair; every
+     * other entry takes one thread.  nsplit fills the block: with nc + nsplit
+     * items the longest loop is about half the longest row. */
     const int *uperm = ent + 7 * K2 + Tn + 2;
-    const bool uact = tid < nc;
-    int ue = 0, ul = 0, um = 0, uam = 0;
+    int nlen1 = 0;   /* trailing entries of row length 1 (|m| = L): not split */
+    for (int i = nc - 1; i >= 0 && nlen1 < 2; i--) {
+        const int mm = ent_m[cstart + uperm[cstart + i]];
+        if ((mm < 0 ? -mm : mm) != L) break;
+        nlen1++;
+    }
+    const int nsplit = nc < FSX_CROSS_BLOCK ? min(FSX_CROSS_BLOCK - nc, nc - nlen1) : 0;
+    const bool uact = tid < nc + nsplit;
+    const bool usplit = tid < 2 * nsplit;
+    int ue = 0, ul = 0, um = 0, uam = 0, uk0 = 0, uk1 = 0;
     if (uact) {
-        ue = uperm[cstart + tid];
+        ue = uperm[cstart + (usplit ? tid >> 1 : tid - nsplit)];
         ul = ent_l[cstart + ue];
         um = ent_m[cstart + ue];
         uam = um < 0 ? -um : um;
+        const int nk = L - uam + 1, h = (nk + 1) >> 1;
+        uk0 = usplit && (tid & 1) ? h : 0;
+        uk1 = usplit && !(tid & 1) ? h : nk;
     }
+    const bool ustore = uact && !(usplit && (tid & 1));
     const int ub0 = uact ? mstart[um + L] - cstart : 0;
     /* T in LDS is stored per |m| block as [l1][l] (consecutive l, i.e.
      * consecutive threads, at consecutive addresses): element (am, l, l1)
@@ -521,7 +541,17 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
     const bool writer = (lane & 7) == 0 && slot < 6;
     const int ngg = (ng + G - 1) / G;   /* gamma2 groups per beta2 */
     const int nbg = nbeta * ngg;
-    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    /* Tiles in nx contiguous ranges of the (q, beta2, gamma2 group) order,
+     * one per XCD: workgroups are dispatched to the XCDs round-robin, so
+     * block b runs on XCD b % nx, and the blocks of one XCD walk one range
+     * together.  At any time they then read the receptor rows of about one
+     * q, which that XCD's L2 can hold.  nx = 1: one range, plain grid
+     * stride.  The grid is a multiple of nx. */
+    const long per = (ntiles + nx - 1) / nx;
+    const int xb = blockIdx.x % nx, gb = gridDim.x / nx;
+    for (long j = blockIdx.x / nx; j < per; j += gb) {
+        const long tile = xb * per + j;
+        if (tile >= ntiles) break;
         const int q = (int)(tile / nbg), bgg = (int)(tile - (long)q * nbg);
         const int b2 = bgg / ngg, g20 = (bgg - b2 * ngg) * G;
         const int nr = ng - g20 < G ? ng - g20 : G;   /* rows of this tile */
@@ -598,16 +628,15 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                 }
             }
             __syncthreads();
-            if (uact) {
+            {
                 const E *Tr = (TST ? sT : Tq) + urow;
                 const E *bp = sB + ub0;
-                const int nk = L - uam + 1;
                 F ur[G][3], ui[G][3];
 #pragma unroll
                 for (int r = 0; r < G; r++)
 #pragma unroll
                     for (int Y = 0; Y < 3; Y++) ur[r][Y] = ui[r][Y] = 0;
-                for (int k = 0; k < nk; k++) {
+                for (int k = uk0; k < uk1; k++) {
                     E tv = Tr[k * ustep];
 #pragma unroll
                     for (int r = 0; r < G; r++)
@@ -619,13 +648,30 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                             ui[r][Y] += tv.re * bv.im - tv.im * bv.re;
                         }
                 }
+                /* split entries: the lower half (even lane) plus the upper
+                 * half (its odd neighbour), in that order */
+                if (nsplit > 0) {
 #pragma unroll
-                for (int r = 0; r < G; r++)
+                    for (int r = 0; r < G; r++)
 #pragma unroll
-                    for (int Y = 0; Y < 3; Y++) {
-                        sU[(r * 3 + Y) * nc + ue].re = ur[r][Y];
-                        sU[(r * 3 + Y) * nc + ue].im = ui[r][Y];
-                    }
+                        for (int Y = 0; Y < 3; Y++) {
+                            const F orr = dpp_mov<FSX_DPP_XOR1>(ur[r][Y]);
+                            const F oi = dpp_mov<FSX_DPP_XOR1>(ui[r][Y]);
+                            if (usplit) {
+                                ur[r][Y] += orr;
+                                ui[r][Y] += oi;
+                            }
+                        }
+                }
+                if (ustore) {
+#pragma unroll
+                    for (int r = 0; r < G; r++)
+#pragma unroll
+                        for (int Y = 0; Y < 3; Y++) {
+                            sU[(r * 3 + Y) * nc + ue].re = ur[r][Y];
+                            sU[(r * 3 + Y) * nc + ue].im = ui[r][Y];
+                        }
+                }
             }
             __syncthreads();
             for (int p = wave; p < cnt; p += nwave) {
@@ -669,10 +715,10 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                     const bool hi = (lane & 8) != 0;
                     const double send = hi ? v[0] : v[1];
                     const double keep = hi ? v[1] : v[0];
-                    v[0] = keep + dpp_f64<FSX_DPP_ROR8>(send);
-                    v[0] += dpp_f64<FSX_DPP_HMIRROR>(v[0]);
-                    v[0] += dpp_f64<FSX_DPP_XOR1>(v[0]);
-                    v[0] += dpp_f64<FSX_DPP_XOR2>(v[0]);
+                    v[0] = keep + dpp_mov<FSX_DPP_ROR8>(send);
+                    v[0] += dpp_mov<FSX_DPP_HMIRROR>(v[0]);
+                    v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
+                    v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 }
                 if (writer) {
                     double x = 2.0 * v[0];
@@ -915,6 +961,7 @@ static int ensure(int slot, size_t bytes)
 }
 
 static int g_ncu = 256;
+static int g_nxcd = 1;   /* XCDs (accelerator complex dies): workgroups go round-robin */
 
 /* launch-shape knobs: environment override for tuning, defaults otherwise */
 static int env_int(const char *name, int dflt, int lo, int hi)
@@ -934,6 +981,9 @@ static int init_once(void)
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (g_ncu < 1) g_ncu = 1;
+    if (hipDeviceGetAttribute(&g_nxcd, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess ||
+        g_nxcd < 1)
+        g_nxcd = 1;
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_init = 1;
     return FSX_OK;
@@ -1029,12 +1079,16 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
             g = (long)g_ncu * per;
         }
         if (g > ntiles) g = ntiles;
+        /* XCD-contiguous tile ranges when the grid divides evenly
+         * (FSX_CROSS_XCD=0 turns them off) */
+        int nx = env_int("FSX_CROSS_XCD", 1, 0, 1) ? g_nxcd : 1;
+        if (nx > 1 && (g % nx != 0 || ntiles < 4L * g)) nx = 1;
         hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(FSX_CROSS_BLOCK), lds, s,
                            (double *)g_buf[B_ACC], (const E *)g_buf[B_RT] + coff[c],
                            (const E *)g_buf[B_BT] + coff[c], (const E *)g_buf[B_T],
                            (const double *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
                            (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
-                           (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, na, ng, zn,
+                           (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
                            c == 0);
     }
 }
@@ -1375,6 +1429,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                              != hipSuccess || fit_bpc < 1))
         fit_bpc = 1;
     long lanes = (long)g_ncu * 256 * fit_bpc;
+    const int fit_blocks = env_int("FSX_FIT_BLOCKS", 0, 0, 1 << 20);
+    if (fit_blocks > 0) lanes = 256L * fit_blocks;
     if (lanes > n) lanes = ((long)n + 255) / 256 * 256;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_NEXT], 1L, (int)lanes);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_TALLY], 4L, 0);
