@@ -349,7 +349,7 @@ k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long 
  * per block: loads run along rows, stores along entries. */
 template <typename E, int TR>
 __global__ void __launch_bounds__(256)
-k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr)
+k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int npk)
 {
     __shared__ E tile[64][TR + 1];
     int bid = blockIdx.x;
@@ -369,7 +369,13 @@ k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr)
     for (int k = threadIdx.x; k < 64 * TR; k += 256) {
         const int j = k >> 6, i = k & 63;
         const int e = e0 + i, r = r0 + j;
-        if (e < cs && r < nrows) out[(((long)q * nrows + r) * 3 + X) * cs + e] = tile[i][j];
+        if (e < cs && r < nrows) {
+            E *d = out + (((long)q * nrows + r) * 3 + X) * cs;
+            const E v = tile[i][j];
+            if (e >= 2 * npk) d[e - npk] = v;
+            else if (e & 1) d[e >> 1].im = v.im;
+            else d[e >> 1].re = v.re;
+        }
     }
 }
 
@@ -484,7 +490,7 @@ __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 5 : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
-        int zn, int first)
+        int zn, int first, int npk)
 {
 #pragma clang fp contract(fast)
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
@@ -531,11 +537,18 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
      * at tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am) */
     const int urow = uact ? (TST ? tpk_row(uam, uam, K) + (ul - uam) : tpk_row(uam, ul, K)) : 0;
     const int ustep = TST ? K - uam : 1;
+    /* pairing slots: with npk > 0 (conjugate-symmetric coefficients, the
+     * chunk of m = 0) the m = 0 entries l = 2i and 2i + 1 share slot i, the
+     * real part of the first (real for even l) and the imaginary part of the
+     * second (imaginary for odd l); their twiddle is 1, so Re(w R U) of the
+     * slot is the sum of the two entries' terms.  Entry e >= 2 npk has slot
+     * e - npk. */
+    const int ns = nc - npk;
     int em[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
         int ec = lane + FSX_WAVE * j;
-        em[j] = ec < nc ? ent_m[cstart + ec] : 0;
+        em[j] = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : 0;
     }
     const int slot = lane >> 3;
     const bool writer = (lane & 7) == 0 && slot < 6;
@@ -584,7 +597,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
-                    if (ec < nc) {
+                    if (ec < ns) {
                         const double *tw = tw2 + 2 * ((long)rec.z * N + em[j] + L);
                         wr[j] = (F)tw[0];
                         wi[j] = (F)tw[1];
@@ -664,12 +677,20 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                         }
                 }
                 if (ustore) {
+                    const int us = ue < 2 * npk ? ue >> 1 : ue - npk;   /* pairing slot */
 #pragma unroll
                     for (int r = 0; r < G; r++)
 #pragma unroll
                         for (int Y = 0; Y < 3; Y++) {
-                            sU[(r * 3 + Y) * nc + ue].re = ur[r][Y];
-                            sU[(r * 3 + Y) * nc + ue].im = ui[r][Y];
+                            E *d = sU + (r * 3 + Y) * nc + us;
+                            if (ue >= 2 * npk) {
+                                d->re = ur[r][Y];
+                                d->im = ui[r][Y];
+                            } else if (ue & 1) {
+                                d->im = ui[r][Y];
+                            } else {
+                                d->re = ur[r][Y];
+                            }
                         }
                 }
             }
@@ -681,7 +702,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
-                    if (ec < nc) {
+                    if (ec < ns) {
                         E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
                         E uv = sUp[ec], ud = sUp[nc + ec], uw = sUp[2 * nc + ec];
                         F w_r = wr[j], w_i = wi[j];
@@ -1027,7 +1048,7 @@ static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
 template <typename E>
 static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct, long nrot,
                           size_t lds, int nb, int ng, int Q, int L, int gc, int C,
-                          const int *csz, const long *coff, int mlo)
+                          const int *csz, const long *coff, int mlo, int npk0)
 {
     hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)tmp, srct,
                        (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
@@ -1041,26 +1062,29 @@ static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct,
         const long nblkT = (long)Q * 3 * nte * ntr;
         if (sizeof(E) == 8)
             hipLaunchKernelGGL((k_tab_transpose<E, 64>), dim3((unsigned)nblkT), dim3(256), 0, s,
-                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr);
+                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
+                               c == 0 ? npk0 : 0);
         else
             hipLaunchKernelGGL((k_tab_transpose<E, 32>), dim3((unsigned)nblkT), dim3(256), 0, s,
-                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr);
+                               (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
+                               c == 0 ? npk0 : 0);
     }
 }
 
 template <typename E, int G>
 static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                            const long *coff, int Tn, long grid, int nrows, int Q, int L,
-                           int nb, int na, int ng, int zn)
+                           int nb, int npk0, int ng, int zn)
 {
     const int tst = (long)Tn * (long)sizeof(E) <= 40 * 1024;
     const long ntiles = (long)Q * nb * ((ng + G - 1) / G);
     for (int c = 0; c < C; c++) {
         if (csz[c] == 0) continue;
-        int jn = (csz[c] + FSX_WAVE - 1) / FSX_WAVE;
+        const int npk = c == 0 ? npk0 : 0;
+        int jn = (csz[c] - npk + FSX_WAVE - 1) / FSX_WAVE;
         void (*kern)(double *, const E *, const E *, const E *, const double *, const int *,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
-                     int, int, int, int, int);
+                     int, int, int, int, int, int);
         if (tst)
             kern = jn <= 1 ? k_cross<1, 1, G, E> : jn <= 2 ? k_cross<2, 1, G, E>
                  : jn <= 3 ? k_cross<3, 1, G, E> : k_cross<4, 1, G, E>;
@@ -1089,20 +1113,20 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                            (const double *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
                            (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
                            (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
-                           c == 0);
+                           c == 0, npk);
     }
 }
 
 /* G = gamma2 rows per tile: FSX_CROSS_ROWS, else 2 */
 template <typename E>
 static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
-                         int Tn, long grid, int nrows, int Q, int L, int nb, int na, int ng,
+                         int Tn, long grid, int nrows, int Q, int L, int nb, int npk0, int ng,
                          int zn)
 {
     const int G = env_int("FSX_CROSS_ROWS", 2, 1, 4);
-    if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
-    else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
-    else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
+    else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
+    else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
 }
 
 /* Entry tables of the m-major coefficient order used by the rotated tables
@@ -1188,6 +1212,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                      spf_conj_symmetric(A->A_re, A->A_im, Q, L) &&
                      spf_conj_symmetric(A->B_re, A->B_im, Q, L);
     const int mlo = half ? 0 : -L;
+    /* half set: the m = 0 entries of the receptor table and of U share
+     * pairing slots two by two (FSX_CROSS_PACK=0 keeps one slot each) */
+    const int npk = half && env_int("FSX_CROSS_PACK", 1, 0, 1) ? K / 2 : 0;
     const size_t es = mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
     hipStream_t s = g_stream;
     /* m-chunks: one launch per chunk of entries; one chunk unless a chunk
@@ -1385,19 +1412,19 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff, mlo);
+                                  L, gc, C, csz, coff, mlo, npk);
         else
             launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff, mlo);
+                                gc, C, csz, coff, mlo, npk);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff, mlo);
+                                  L, gc, C, csz, coff, mlo, 0);
         else
             launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff, mlo);
+                                gc, C, csz, coff, mlo, 0);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
@@ -1415,8 +1442,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipEventRecord(ev[3], s));
     /* per-pose cross terms, one launch per m-chunk, accumulated in order */
     long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
-    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
-    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, na, ng, zn);
+    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk, ng, zn);
+    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk, ng, zn);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue: as many lanes as are resident at once
@@ -1481,14 +1508,16 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     st->cross_chunks = C;
     /* useful FP64 flops of the cross-term launches: 8 per complex
      * multiply-add of the translated ligand (3 parts, sum_e (L+1-|m_e|)
-     * terms per item) and 54 per (pose, q, entry) of the pairing */
+     * terms per item) and 54 per (pose, q, pairing slot); the npk packed
+     * slots carry two m = 0 entries whose other halves are zero */
     double tpair = 0.0;
     for (int e = 0; e < ne; e++) tpair += K - (ent[K2 + e] < 0 ? -ent[K2 + e] : ent[K2 + e]);
-    st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * ne * 54.0;
-    /* algorithmic bytes: per pose and q one receptor row (3 parts) and the
-     * six-term accumulator (read and written by every chunk after the
-     * first); per item one ligand row */
-    st->cross_bytes = (double)nvalid * Q * (3.0 * ne * (double)es + (2.0 * C - 1.0) * 48.0)
+    const double nslot = (double)(ne - npk);
+    st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * nslot * 54.0;
+    /* algorithmic bytes: per pose and q one receptor row (3 parts, one
+     * value per slot) and the six-term accumulator (read and written by
+     * every chunk after the first); per item one ligand row */
+    st->cross_bytes = (double)nvalid * Q * (3.0 * nslot * (double)es + (2.0 * C - 1.0) * 48.0)
                     + (double)items * 3.0 * ne * (double)es;
     st->cross_entries = ne;
     }
