@@ -198,6 +198,18 @@ __device__ __forceinline__ double tr_swap16(double a, double b)
     const double r1 = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
     return r0 + r1;
 }
+__device__ __forceinline__ float tr_swap32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a),
+                                                    __builtin_bit_cast(unsigned, b), false, false);
+    return __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+}
+__device__ __forceinline__ float tr_swap16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a),
+                                                    __builtin_bit_cast(unsigned, b), false, false);
+    return __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+}
 /* a double moved between lanes by a DPP pattern (row_ror:8 = lane xor 8,
  * row_half_mirror = lane 7 - i in each group of 8, quad_perm) */
 template <int CTRL>
@@ -487,7 +499,7 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  * LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 5 : 1)
-k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
+k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
         int zn, int first, int npk)
@@ -598,9 +610,9 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                 for (int j = 0; j < J; j++) {
                     int ec = lane + FSX_WAVE * j;
                     if (ec < ns) {
-                        const double *tw = tw2 + 2 * ((long)rec.z * N + em[j] + L);
-                        wr[j] = (F)tw[0];
-                        wi[j] = (F)tw[1];
+                        const E w = tw2[(long)rec.z * N + em[j] + L];
+                        wr[j] = w.re;
+                        wi[j] = w.im;
 #pragma unroll
                         for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + ec];
                     }
@@ -723,8 +735,8 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
                         va[5] += xr * uw.re; va[5] -= xi * uw.im;
                     }
                 }
-                /* the cross-lane sums run in FP64 */
-                double v[8] = {va[0], va[1], va[2], va[3], va[4], va[5], 0.0, 0.0};
+                /* the cross-lane sums run in the contraction's type */
+                F v[8] = {va[0], va[1], va[2], va[3], va[4], va[5], 0, 0};
                 /* transpose-reduce over the 64 lanes: after the three halving
                  * steps lane bits 5,4,3 select the slot, then a sum over the
                  * 8 lanes of bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
@@ -734,8 +746,8 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const double *tw2,
 #pragma unroll
                     for (int i = 0; i < 2; i++) v[i] = tr_swap16(v[i], v[i + 2]);
                     const bool hi = (lane & 8) != 0;
-                    const double send = hi ? v[0] : v[1];
-                    const double keep = hi ? v[1] : v[0];
+                    const F send = hi ? v[0] : v[1];
+                    const F keep = hi ? v[1] : v[0];
                     v[0] = keep + dpp_mov<FSX_DPP_ROR8>(send);
                     v[0] += dpp_mov<FSX_DPP_HMIRROR>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
@@ -1082,7 +1094,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         if (csz[c] == 0) continue;
         const int npk = c == 0 ? npk0 : 0;
         int jn = (csz[c] - npk + FSX_WAVE - 1) / FSX_WAVE;
-        void (*kern)(double *, const E *, const E *, const E *, const double *, const int *,
+        void (*kern)(double *, const E *, const E *, const E *, const E *, const int *,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
                      int, int, int, int, int, int);
         if (tst)
@@ -1110,7 +1122,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(FSX_CROSS_BLOCK), lds, s,
                            (double *)g_buf[B_ACC], (const E *)g_buf[B_RT] + coff[c],
                            (const E *)g_buf[B_BT] + coff[c], (const E *)g_buf[B_T],
-                           (const double *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
+                           (const E *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
                            (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
                            (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
                            c == 0, npk);
@@ -1284,6 +1296,11 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
             tw2[2 * ((long)a2 * N + m + L)] = f * A->tw_a[2 * k];
             tw2[2 * ((long)a2 * N + m + L) + 1] = f * A->tw_a[2 * k + 1];
         }
+    /* the mixed form reads them as FP32 pairs, in place */
+    if (A->flags & FSX_MIXED_FP32) {
+        float *f32 = (float *)tw2;
+        for (long i = 0; i < 2L * na * N; i++) f32[i] = (float)tw2[i];
+    }
     const int *csz = ent + 4 * K2, *cst = ent + 6 * K2;
     coff[0] = 0;
     for (int c = 0; c < C; c++) coff[c + 1] = coff[c] + (long)Q * nrows * 3 * csz[c];
