@@ -494,11 +494,12 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  * over the chunk's entries e = (l, m), stored (first chunk) or added into
  * acc[(s*Q + q)*6 + k]. */
 #define FSX_CROSS_BLOCK 256
-/* FP32 tables: at most 96 VGPRs, so that five blocks of four wavefronts fit
- * a CU (with fewer registers than 96 the kernel spills); FP64 tables: the
- * LDS of three blocks fills a CU, no register bound */
+/* FP32 tables: at most 96 VGPRs with two gamma2 rows per tile, so that five
+ * blocks of four wavefronts fit a CU (with fewer registers than 96 the kernel
+ * spills); with four rows the LDS admits four blocks, and 128 VGPRs; FP64
+ * tables: the LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E>
-__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 5 : 1)
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? (G >= 4 ? 4 : 5) : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
@@ -824,11 +825,17 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
       int gcache)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double sG[];   /* [Q][256] when gcache */
+    extern __shared__ double sG[];   /* [Q][256] when gcache, then a [6][Q] and q [Q] */
     const int nvalid = *nvalid_p;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
+    /* the moments and the q grid, read by every lane at the same address,
+     * from LDS */
+    double *sA = sG + (gcache ? (long)Q * 256 : 0), *sQ = sA + 6 * Q;
+    for (int i = threadIdx.x; i < 6 * Q; i += blockDim.x) sA[i] = a[i];
+    for (int i = threadIdx.x; i < Q; i += blockDim.x) sQ[i] = qvals[i];
+    __syncthreads();
     fsx_fitpar fp;
-    fp.a = a; fp.q = qvals; fp.qnum = Q; fp.mult = mult;
+    fp.a = sA; fp.q = sQ; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
     /* profiles already rescaled in place (k_fit_prep) */
@@ -1129,13 +1136,15 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
     }
 }
 
-/* G = gamma2 rows per tile: FSX_CROSS_ROWS, else 2 */
+/* G = gamma2 rows per tile: FSX_CROSS_ROWS, else 4 for FP32 tables (the
+ * translation block and the pose loop shared by more rows) and 2 for FP64
+ * tables (LDS-bound) */
 template <typename E>
 static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
                          int Tn, long grid, int nrows, int Q, int L, int nb, int npk0, int ng,
                          int zn)
 {
-    const int G = env_int("FSX_CROSS_ROWS", 2, 1, 4);
+    const int G = env_int("FSX_CROSS_ROWS", sizeof(E) == 8 ? 4 : 2, 1, 4);
     if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
     else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
     else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
@@ -1467,7 +1476,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
      * (the fit state lives in registers, one 256-thread block per CU) */
     const int gcache = env_int("FSX_FIT_GCACHE", 1, 0, 1) &&
                        (size_t)Q * 256 * sizeof(double) <= FSX_FIT_GCACHE;
-    const size_t lds_fit = gcache ? (size_t)Q * 256 * sizeof(double) : 0;
+    const size_t lds_fit = (gcache ? (size_t)Q * 256 * sizeof(double) : 0) + 7 * (size_t)Q * sizeof(double);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, lds_fit)
                              != hipSuccess || fit_bpc < 1))
