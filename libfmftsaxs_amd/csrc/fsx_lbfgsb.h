@@ -121,7 +121,7 @@ FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 
 /* Terms at q = i0 .. i0+nb-1 (nb <= FSX_QB) into blk: all raw loads first,
  * then (c + v) * scale per value exactly as fsx_ct_get. */
-#define FSX_QB 4
+#define FSX_QB 8
 FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
 {
     if (!t->c) {
