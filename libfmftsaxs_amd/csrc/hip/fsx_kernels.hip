@@ -134,7 +134,7 @@ __host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
  * thread per (z, q, am, l, l1) of the full cube; valid entries are stored in
  * the packed layout. */
 template <typename E>
-__global__ void k_tmat(E *T, const double *bes, const double *ds, int znum, int Q, int L)
+__global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, int znum, int Q, int L)
 {
 #pragma clang fp contract(off)
     const int K = L + 1, N = 2 * L + 1;
@@ -156,10 +156,12 @@ __global__ void k_tmat(E *T, const double *bes, const double *ds, int znum, int 
         re += ipr[p % 4] * v;
         im += ipi[p % 4] * v;
     }
-    E out;
-    out.re = re;
-    out.im = im;
-    T[zq * tpk_sq(K) + tpk_row(m, l, K) + (l1 - m)] = out;
+    /* only p of the parity of l + l1 contribute (the 3j symbol (l p l1; 0 0 0)
+     * vanishes exactly otherwise), so T = i^(l1 - l) tau with tau real, and
+     * the other part is an exact zero: stored is tau = i^(l - l1) T */
+    const int ph = ((l - l1) % 4 + 4) % 4;
+    const double tau = ph == 0 ? re : ph == 1 ? -im : ph == 2 ? -re : im;
+    T[zq * tpk_sq(K) + tpk_row(m, l, K) + (l1 - m)] = (decltype(E::re))tau;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -361,7 +363,8 @@ k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long 
  * per block: loads run along rows, stores along entries. */
 template <typename E, int TR>
 __global__ void __launch_bounds__(256)
-k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int npk)
+k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int npk,
+                const int *lrot)
 {
     __shared__ E tile[64][TR + 1];
     int bid = blockIdx.x;
@@ -383,7 +386,13 @@ k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int n
         const int e = e0 + i, r = r0 + j;
         if (e < cs && r < nrows) {
             E *d = out + (((long)q * nrows + r) * 3 + X) * cs;
-            const E v = tile[i][j];
+            E v = tile[i][j];
+            if (lrot) {   /* times i^(-l) of the entry */
+                const int ph = lrot[e] & 3;
+                const decltype(v.re) a = v.re, b = v.im;
+                v.re = ph == 0 ? a : ph == 1 ? b : ph == 2 ? -a : -b;
+                v.im = ph == 0 ? b : ph == 1 ? -a : ph == 2 ? -b : a;
+            }
             if (e >= 2 * npk) d[e - npk] = v;
             else if (e & 1) d[e >> 1].im = v.im;
             else d[e >> 1].re = v.re;
@@ -500,7 +509,7 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  * tables: the LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? (G >= 4 ? 4 : 5) : 1)
-k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
+k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
         int zn, int first, int npk)
@@ -509,9 +518,12 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
     extern __shared__ double smem_raw[];
     E *smem = reinterpret_cast<E *>(smem_raw);
-    E *sB = smem;                /* [G][3][nc] */
+    /* ligand rows: FP32 tables [nc][G][3] (the 3G values of one entry
+     * together, read as 16-byte pairs), FP64 tables [G][3][nc] */
+    constexpr bool BIL = sizeof(E) == 8;
+    E *sB = smem;
     E *sU = smem + 3 * G * nc;   /* [G][3][nc] */
-    E *sT = smem + 6 * G * nc;   /* T block, transposed per |m| (TST) */
+    F *sT = reinterpret_cast<F *>(smem + 6 * G * nc);   /* T block, transposed per |m| (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
     const int *tmap = ent + 7 * K2;
@@ -594,7 +606,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
             const int cnt = cum[G];
             if (cnt == 0) continue;
             const int start = offs[key];
-            const E *Tq = T + ((long)z * Q + q) * Tn;
+            const F *Tq = T + ((long)z * Q + q) * Tn;
             /* pose loader: receptor row entries, twiddles, accumulator, row */
             E rr[3][J];
             F wr[J], wi[J];
@@ -624,7 +636,13 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
+                /* global (row r, part Y, entry e) -> LDS e * 3G + 3r + Y (BIL) */
                 const int nb3 = 3 * nc * nr;
+                auto bdst = [&](int b) {
+                    if (!BIL) return b;
+                    const int rY = b / nc;
+                    return (b - rY * nc) * (3 * G) + rY;
+                };
                 for (int b0 = tid; b0 < nb3; b0 += 4 * FSX_CROSS_BLOCK) {
                     const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
                     const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
@@ -632,10 +650,10 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
                     E t1 = Bq[i1 < nb3 ? i1 : b0];
                     E t2 = Bq[i2 < nb3 ? i2 : b0];
                     E t3 = Bq[i3 < nb3 ? i3 : b0];
-                    sB[b0] = t0;
-                    if (i1 < nb3) sB[i1] = t1;
-                    if (i2 < nb3) sB[i2] = t2;
-                    if (i3 < nb3) sB[i3] = t3;
+                    sB[bdst(b0)] = t0;
+                    if (i1 < nb3) sB[bdst(i1)] = t1;
+                    if (i2 < nb3) sB[bdst(i2)] = t2;
+                    if (i3 < nb3) sB[bdst(i3)] = t3;
                 }
                 staged = true;
             }
@@ -645,7 +663,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
                     const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
                     const int j1 = i1 < Tn ? i1 : b0, j2 = i2 < Tn ? i2 : b0;
                     const int j3 = i3 < Tn ? i3 : b0;
-                    E t0 = Tq[b0], t1 = Tq[j1], t2 = Tq[j2], t3 = Tq[j3];
+                    F t0 = Tq[b0], t1 = Tq[j1], t2 = Tq[j2], t3 = Tq[j3];
                     int d0 = tmap[b0], d1 = tmap[j1], d2 = tmap[j2], d3 = tmap[j3];
                     sT[d0] = t0;
                     if (i1 < Tn) sT[d1] = t1;
@@ -655,23 +673,28 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
             }
             __syncthreads();
             {
-                const E *Tr = (TST ? sT : Tq) + urow;
-                const E *bp = sB + ub0;
+                /* T = i^(l1 - l) tau with tau real and the ligand rows stored
+                 * as i^(-l1) B, so that conj(T) B = i^l tau (i^(-l1) B): a
+                 * real-by-complex sum, turned by i^l at the store */
+                const F *Tr = (TST ? sT : Tq) + urow;
+                const E *bp = sB + ub0 * (BIL ? 3 * G : 1);
                 F ur[G][3], ui[G][3];
 #pragma unroll
                 for (int r = 0; r < G; r++)
 #pragma unroll
                     for (int Y = 0; Y < 3; Y++) ur[r][Y] = ui[r][Y] = 0;
                 for (int k = uk0; k < uk1; k++) {
-                    E tv = Tr[k * ustep];
+                    /* the step's LDS reads first, then its FMAs */
+                    const F tv = Tr[k * ustep];
+                    E bv[3 * G];
+#pragma unroll
+                    for (int i = 0; i < 3 * G; i++) bv[i] = BIL ? bp[k * (3 * G) + i] : bp[i * nc + k];
 #pragma unroll
                     for (int r = 0; r < G; r++)
 #pragma unroll
                         for (int Y = 0; Y < 3; Y++) {
-                            E bv = bp[(r * 3 + Y) * nc + k];
-                            /* conj(t) * b */
-                            ur[r][Y] += tv.re * bv.re + tv.im * bv.im;
-                            ui[r][Y] += tv.re * bv.im - tv.im * bv.re;
+                            ur[r][Y] += tv * bv[r * 3 + Y].re;
+                            ui[r][Y] += tv * bv[r * 3 + Y].im;
                         }
                 }
                 /* split entries: the lower half (even lane) plus the upper
@@ -691,18 +714,22 @@ k_cross(double *acc, const E *Rc, const E *Bc, const E *T, const E *tw2,
                 }
                 if (ustore) {
                     const int us = ue < 2 * npk ? ue >> 1 : ue - npk;   /* pairing slot */
+                    const int ph = ul & 3;
 #pragma unroll
                     for (int r = 0; r < G; r++)
 #pragma unroll
                         for (int Y = 0; Y < 3; Y++) {
+                            const F a = ur[r][Y], b = ui[r][Y];   /* U = i^l (a + i b) */
+                            const F u_re = ph == 0 ? a : ph == 1 ? -b : ph == 2 ? -a : b;
+                            const F u_im = ph == 0 ? b : ph == 1 ? a : ph == 2 ? -b : -a;
                             E *d = sU + (r * 3 + Y) * nc + us;
                             if (ue >= 2 * npk) {
-                                d->re = ur[r][Y];
-                                d->im = ui[r][Y];
+                                d->re = u_re;
+                                d->im = u_im;
                             } else if (ue & 1) {
-                                d->im = ui[r][Y];
+                                d->im = u_im;
                             } else {
-                                d->re = ur[r][Y];
+                                d->re = u_re;
                             }
                         }
                 }
@@ -1060,14 +1087,15 @@ static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t);
 template <typename E>
 static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
 {
-    hipLaunchKernelGGL(k_tmat<E>, dim3(nblk(nt, 256)), dim3(256), 0, s, (E *)T,
+    hipLaunchKernelGGL(k_tmat<E>, dim3(nblk(nt, 256)), dim3(256), 0, s, (decltype(E::re) *)T,
                        (const double *)g_buf[B_BES], (const double *)g_buf[B_DS], zn, Q, L);
 }
 
 template <typename E>
 static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct, long nrot,
                           size_t lds, int nb, int ng, int Q, int L, int gc, int C,
-                          const int *csz, const long *coff, int mlo, int npk0)
+                          const int *csz, const int *cst, const long *coff, int mlo, int npk0,
+                          int lrot)
 {
     hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)tmp, srct,
                        (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
@@ -1082,11 +1110,11 @@ static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct,
         if (sizeof(E) == 8)
             hipLaunchKernelGGL((k_tab_transpose<E, 64>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0);
+                               c == 0 ? npk0 : 0, lrot ? (const int *)g_buf[B_ENT] + cst[c] : NULL);
         else
             hipLaunchKernelGGL((k_tab_transpose<E, 32>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0);
+                               c == 0 ? npk0 : 0, lrot ? (const int *)g_buf[B_ENT] + cst[c] : NULL);
     }
 }
 
@@ -1095,13 +1123,13 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                            const long *coff, int Tn, long grid, int nrows, int Q, int L,
                            int nb, int npk0, int ng, int zn)
 {
-    const int tst = (long)Tn * (long)sizeof(E) <= 40 * 1024;
+    const int tst = (long)Tn * (long)(sizeof(E) / 2) <= 40 * 1024;
     const long ntiles = (long)Q * nb * ((ng + G - 1) / G);
     for (int c = 0; c < C; c++) {
         if (csz[c] == 0) continue;
         const int npk = c == 0 ? npk0 : 0;
         int jn = (csz[c] - npk + FSX_WAVE - 1) / FSX_WAVE;
-        void (*kern)(double *, const E *, const E *, const E *, const E *, const int *,
+        void (*kern)(double *, const E *, const E *, const decltype(E::re) *, const E *, const int *,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
                      int, int, int, int, int, int);
         if (tst)
@@ -1110,7 +1138,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         else
             kern = jn <= 1 ? k_cross<1, 0, G, E> : jn <= 2 ? k_cross<2, 0, G, E>
                  : jn <= 3 ? k_cross<3, 0, G, E> : k_cross<4, 0, G, E>;
-        size_t lds = sizeof(E) * (6 * (size_t)G * csz[c] + (tst ? Tn : 0));
+        size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] + (tst ? (sizeof(E) / 2) * Tn : 0);
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
         long g = grid;
@@ -1128,7 +1156,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         if (nx > 1 && (g % nx != 0 || ntiles < 4L * g)) nx = 1;
         hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(FSX_CROSS_BLOCK), lds, s,
                            (double *)g_buf[B_ACC], (const E *)g_buf[B_RT] + coff[c],
-                           (const E *)g_buf[B_BT] + coff[c], (const E *)g_buf[B_T],
+                           (const E *)g_buf[B_BT] + coff[c], (const decltype(E::re) *)g_buf[B_T],
                            (const E *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
                            (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
                            (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
@@ -1438,19 +1466,19 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff, mlo, npk);
+                                  L, gc, C, csz, cst, coff, mlo, npk, 0);
         else
             launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff, mlo, npk);
+                                gc, C, csz, cst, coff, mlo, npk, 0);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, coff, mlo, 0);
+                                  L, gc, C, csz, cst, coff, mlo, 0, 1);
         else
             launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, coff, mlo, 0);
+                                gc, C, csz, cst, coff, mlo, 0, 1);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
