@@ -364,7 +364,7 @@ k_rotate(E *out, const cplx *srct, const double *dW, const int *ent, const long 
 template <typename E, int TR>
 __global__ void __launch_bounds__(256)
 k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int npk,
-                const int *lrot)
+                const int *lrot, int lsgn)
 {
     __shared__ E tile[64][TR + 1];
     int bid = blockIdx.x;
@@ -387,14 +387,15 @@ k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int n
         if (e < cs && r < nrows) {
             E *d = out + (((long)q * nrows + r) * 3 + X) * cs;
             E v = tile[i][j];
-            if (lrot) {   /* times i^(-l) of the entry */
-                const int ph = lrot[e] & 3;
+            if (lsgn != 0) {   /* times i^(lsgn l), l of the entry */
+                const int ph = (lsgn > 0 ? lrot[e] : -lrot[e]) & 3;
                 const decltype(v.re) a = v.re, b = v.im;
-                v.re = ph == 0 ? a : ph == 1 ? b : ph == 2 ? -a : -b;
-                v.im = ph == 0 ? b : ph == 1 ? -a : ph == 2 ? -b : a;
+                v.re = ph == 0 ? a : ph == 1 ? -b : ph == 2 ? -a : b;
+                v.im = ph == 0 ? b : ph == 1 ? a : ph == 2 ? -b : -a;
             }
+            /* packed m = 0 pairs: i^l R_l0 is real for every l */
             if (e >= 2 * npk) d[e - npk] = v;
-            else if (e & 1) d[e >> 1].im = v.im;
+            else if (e & 1) d[e >> 1].im = v.re;
             else d[e >> 1].re = v.re;
         }
     }
@@ -674,8 +675,8 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             __syncthreads();
             {
                 /* T = i^(l1 - l) tau with tau real and the ligand rows stored
-                 * as i^(-l1) B, so that conj(T) B = i^l tau (i^(-l1) B): a
-                 * real-by-complex sum, turned by i^l at the store */
+                 * as i^(-l1) B, so that i^(-l) conj(T) B = tau (i^(-l1) B): a
+                 * real-by-complex sum */
                 const F *Tr = (TST ? sT : Tq) + urow;
                 const E *bp = sB + ub0 * (BIL ? 3 * G : 1);
                 F ur[G][3], ui[G][3];
@@ -712,24 +713,24 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                             }
                         }
                 }
+                /* stored is i^(-l) U (the receptor rows hold i^l R, so the
+                 * pairing's products are unchanged); in a packed m = 0 slot
+                 * both values are real, the second one negated so that
+                 * Re(R U) of the slot adds the two entries' products */
                 if (ustore) {
                     const int us = ue < 2 * npk ? ue >> 1 : ue - npk;   /* pairing slot */
-                    const int ph = ul & 3;
 #pragma unroll
                     for (int r = 0; r < G; r++)
 #pragma unroll
                         for (int Y = 0; Y < 3; Y++) {
-                            const F a = ur[r][Y], b = ui[r][Y];   /* U = i^l (a + i b) */
-                            const F u_re = ph == 0 ? a : ph == 1 ? -b : ph == 2 ? -a : b;
-                            const F u_im = ph == 0 ? b : ph == 1 ? a : ph == 2 ? -b : -a;
                             E *d = sU + (r * 3 + Y) * nc + us;
                             if (ue >= 2 * npk) {
-                                d->re = u_re;
-                                d->im = u_im;
+                                d->re = ur[r][Y];
+                                d->im = ui[r][Y];
                             } else if (ue & 1) {
-                                d->im = u_im;
+                                d->im = -ur[r][Y];
                             } else {
-                                d->re = u_re;
+                                d->re = ur[r][Y];
                             }
                         }
                 }
@@ -1110,11 +1111,11 @@ static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct,
         if (sizeof(E) == 8)
             hipLaunchKernelGGL((k_tab_transpose<E, 64>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0, lrot ? (const int *)g_buf[B_ENT] + cst[c] : NULL);
+                               c == 0 ? npk0 : 0, (const int *)g_buf[B_ENT] + cst[c], lrot);
         else
             hipLaunchKernelGGL((k_tab_transpose<E, 32>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0, lrot ? (const int *)g_buf[B_ENT] + cst[c] : NULL);
+                               c == 0 ? npk0 : 0, (const int *)g_buf[B_ENT] + cst[c], lrot);
     }
 }
 
@@ -1466,19 +1467,19 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                            (const double *)g_buf[B_TWG], ng, Q, L, 0);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, cst, coff, mlo, npk, 0);
+                                  L, gc, C, csz, cst, coff, mlo, npk, 1);
         else
             launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, cst, coff, mlo, npk, 0);
+                                gc, C, csz, cst, coff, mlo, npk, 1);
         hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
                            (const double *)dB, (const double *)(dB + spf),
                            (const double *)g_buf[B_TWG], ng, Q, L, 1);
         if (mixed)
             launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, cst, coff, mlo, 0, 1);
+                                  L, gc, C, csz, cst, coff, mlo, 0, -1);
         else
             launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, cst, coff, mlo, 0, 1);
+                                gc, C, csz, cst, coff, mlo, 0, -1);
     }
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[2], s));
