@@ -508,7 +508,7 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  * blocks of four wavefronts fit a CU (with fewer registers than 96 the kernel
  * spills); with four rows the LDS admits four blocks, and 128 VGPRs; FP64
  * tables: the LDS of three blocks fills a CU, no register bound */
-template <int J, int TST, int G, typename E>
+template <int J, int TST, int G, typename E, int PW>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? (G >= 4 ? 4 : 5) : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
@@ -570,14 +570,20 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
      * slot is the sum of the two entries' terms.  Entry e >= 2 npk has slot
      * e - npk. */
     const int ns = nc - npk;
+    /* PW poses per wavefront, LP lanes each: lane sl of a pose takes slots
+     * sl, sl + LP, ... (J per lane) */
+    constexpr int LP = FSX_WAVE / PW;
+    const int sl = lane % LP, half = lane / LP;
     int em[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
-        int ec = lane + FSX_WAVE * j;
+        int ec = sl + LP * j;
         em[j] = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : 0;
     }
-    const int slot = lane >> 3;
-    const bool writer = (lane & 7) == 0 && slot < 6;
+    /* after the lane reduction: one pose: lanes 0, 8, .., 40 hold the six
+     * sums; two poses: lanes 0, 4, .., 20 of each half */
+    const int slot = PW == 1 ? lane >> 3 : (lane >> 2) & 7;
+    const bool writer = (PW == 1 ? (lane & 7) == 0 : (lane & 3) == 0) && slot < 6;
     const int ngg = (ng + G - 1) / G;   /* gamma2 groups per beta2 */
     const int nbg = nbeta * ngg;
     /* Tiles in nx contiguous ranges of the (q, beta2, gamma2 group) order,
@@ -622,7 +628,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 const E *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    int ec = lane + FSX_WAVE * j;
+                    int ec = sl + LP * j;
                     if (ec < ns) {
                         const E w = tw2[(long)rec.z * N + em[j] + L];
                         wr[j] = w.re;
@@ -633,7 +639,9 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 }
                 old = (!first && writer) ? acc[((long)s * Q + q) * 6 + slot] : 0.0;
             };
-            if (wave < cnt) load_pose(wave);
+            /* one pose per wavefront: its first receptor row is in flight
+             * while U is formed (two poses: loaded after U, registers) */
+            if (PW == 1 && wave < cnt) load_pose(wave);
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
@@ -736,13 +744,17 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 }
             }
             __syncthreads();
-            for (int p = wave; p < cnt; p += nwave) {
-                if (p != wave) load_pose(p);
+            for (int pp = wave; PW * pp < cnt; pp += nwave) {
+                /* this lane's pose: PW consecutive poses per wavefront (a half
+                 * past the last pose repeats it and does not write) */
+                const int p = PW * pp + half;
+                const bool act = p < cnt;
+                if (PW > 1 || pp != wave) load_pose(min(p, cnt - 1));
                 const E *sUp = sU + prow * 3 * nc;
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    int ec = lane + FSX_WAVE * j;
+                    int ec = sl + LP * j;
                     if (ec < ns) {
                         E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
                         E uv = sUp[ec], ud = sUp[nc + ec], uw = sUp[2 * nc + ec];
@@ -766,10 +778,11 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 }
                 /* the cross-lane sums run in the contraction's type */
                 F v[8] = {va[0], va[1], va[2], va[3], va[4], va[5], 0, 0};
-                /* transpose-reduce over the 64 lanes: after the three halving
-                 * steps lane bits 5,4,3 select the slot, then a sum over the
-                 * 8 lanes of bits 2,1,0; lanes 0, 8, .., 40 hold slots 0..5 */
-                {
+                if (PW == 1) {
+                    /* transpose-reduce over the 64 lanes: after the three
+                     * halving steps lane bits 5,4,3 select the slot, then a
+                     * sum over the 8 lanes of bits 2,1,0; lanes 0, 8, .., 40
+                     * hold slots 0..5 */
 #pragma unroll
                     for (int i = 0; i < 4; i++) v[i] = tr_swap32(v[i], v[i + 4]);
 #pragma unroll
@@ -781,8 +794,28 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     v[0] += dpp_mov<FSX_DPP_HMIRROR>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
+                } else {
+                    /* the same within each 32-lane half (one pose each): after
+                     * the halving steps over lane bits 4 (16-row swap), 3
+                     * (row_ror:8) and 2 (row_half_mirror: lane j <-> 7 - j)
+                     * those bits select the slot, then a sum over the quad of
+                     * bits 1,0; lanes 0, 4, .., 20 of each half hold slots 0..5 */
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] = tr_swap16(v[i], v[i + 4]);
+                    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const F send = b3 ? v[i] : v[i + 2];
+                        const F keep = b3 ? v[i + 2] : v[i];
+                        v[i] = keep + dpp_mov<FSX_DPP_ROR8>(send);
+                    }
+                    const F send = b2 ? v[0] : v[1];
+                    const F keep = b2 ? v[1] : v[0];
+                    v[0] = keep + dpp_mov<FSX_DPP_HMIRROR>(send);
+                    v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
+                    v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 }
-                if (writer) {
+                if (writer && act) {
                     double x = 2.0 * v[0];
                     acc[((long)s * Q + q) * 6 + slot] = first ? x : old + x;
                 }
@@ -1129,16 +1162,26 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
     for (int c = 0; c < C; c++) {
         if (csz[c] == 0) continue;
         const int npk = c == 0 ? npk0 : 0;
-        int jn = (csz[c] - npk + FSX_WAVE - 1) / FSX_WAVE;
+        const int ns = csz[c] - npk;   /* pairing slots */
+        /* FP32 tables: two poses per wavefront (32 lanes each) when the
+         * slots fit four per lane (FSX_CROSS_PW=1: one pose per wavefront) */
+        const int pw = sizeof(E) == 8 && ns <= 4 * (FSX_WAVE / 2) &&
+                       env_int("FSX_CROSS_PW", 2, 1, 2) == 2 ? 2 : 1;
+        int jn = (ns + FSX_WAVE / pw - 1) / (FSX_WAVE / pw);
         void (*kern)(double *, const E *, const E *, const decltype(E::re) *, const E *, const int *,
                      int, int, const int *, const int *, const int4 *, long, int, int, int,
                      int, int, int, int, int, int);
-        if (tst)
-            kern = jn <= 1 ? k_cross<1, 1, G, E> : jn <= 2 ? k_cross<2, 1, G, E>
-                 : jn <= 3 ? k_cross<3, 1, G, E> : k_cross<4, 1, G, E>;
+        if (pw == 2)
+            kern = tst ? (jn <= 1 ? k_cross<1, 1, G, E, 2> : jn <= 2 ? k_cross<2, 1, G, E, 2>
+                          : jn <= 3 ? k_cross<3, 1, G, E, 2> : k_cross<4, 1, G, E, 2>)
+                       : (jn <= 1 ? k_cross<1, 0, G, E, 2> : jn <= 2 ? k_cross<2, 0, G, E, 2>
+                          : jn <= 3 ? k_cross<3, 0, G, E, 2> : k_cross<4, 0, G, E, 2>);
+        else if (tst)
+            kern = jn <= 1 ? k_cross<1, 1, G, E, 1> : jn <= 2 ? k_cross<2, 1, G, E, 1>
+                 : jn <= 3 ? k_cross<3, 1, G, E, 1> : k_cross<4, 1, G, E, 1>;
         else
-            kern = jn <= 1 ? k_cross<1, 0, G, E> : jn <= 2 ? k_cross<2, 0, G, E>
-                 : jn <= 3 ? k_cross<3, 0, G, E> : k_cross<4, 0, G, E>;
+            kern = jn <= 1 ? k_cross<1, 0, G, E, 1> : jn <= 2 ? k_cross<2, 0, G, E, 1>
+                 : jn <= 3 ? k_cross<3, 0, G, E, 1> : k_cross<4, 0, G, E, 1>;
         size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] + (tst ? (sizeof(E) / 2) * Tn : 0);
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
