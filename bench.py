@@ -32,7 +32,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60,
+# The bench workloads (synthetic dimers; the default is the one the headline
+# value is quoted on).
+WORKLOADS = {
+    "standard": dict(npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60, zstep=1.0,
+                     nbeta=19, ngrid=64, n_rec=3000, n_lig=1500),
+    # large single complex: L = 25, 81 q on [0, 0.6], z in 0.5 A steps over
+    # 20..70 A, 5 degree betas, 128-point alpha/gamma grids, 5e6 poses
+    "large": dict(npose=5000000, L=25, qnum=81, qmax=0.6, zlo=20, zhi=70, zstep=0.5,
+                  nbeta=37, ngrid=128, n_rec=10000, n_lig=4000),
+    # the small CPU-runnable case: L = 10, 31 q on [0, 0.3], z 20..50 A, the
+    # native grid of L = 10, 2000 poses
+    "small": dict(npose=2000, L=10, qnum=31, qmax=0.3, zlo=20, zhi=50, zstep=1.0,
+                  nbeta=11, ngrid=21, n_rec=1200, n_lig=800),
+}
+
+
+def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60, zstep=1.0,
                    nbeta=19, ngrid=64, n_rec=3000, n_lig=1500, c1=(0.95, 1.05), c2=(0.0, 4.0)):
     import libfmftsaxs_amd as F
     from libfmftsaxs_amd import synthetic as S
@@ -41,7 +57,7 @@ def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60
     q = F.mkarray(0.0, qmax, qnum)
     Ar, Ai, rmA = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
     Br, Bi, rmB = F.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
-    zv = np.arange(float(zlo), float(zhi) + 0.5, 1.0)
+    zv = np.arange(float(zlo), float(zhi) + 0.5 * zstep, zstep)
     poses = S.random_grid_poses(npose, zv.size, nbeta, ngrid, ngrid, 3001 + seed)
     rm = (rmA * rec.natoms + rmB * lig.natoms) / (rec.natoms + lig.natoms)
     # experimental curve from the first pose
@@ -122,7 +138,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--poses", type=int, default=210000)
+    ap.add_argument("--poses", type=int, default=0, help="pose count (default: the workload's)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="standard",
+                    help="standard (the headline workload), large (L = 25 single complex) "
+                         "or small (the CPU-runnable case)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true",
                     help="time only the mixed-precision form (FP32 contraction, FP64 fit)")
@@ -150,7 +169,10 @@ def main():
     import libfmftsaxs_amd as F
     if F.device_count() < 1:
         raise SystemExit("no HIP device")
-    w = build_workload(npose=args.poses)
+    wk = dict(WORKLOADS[args.workload])
+    if args.poses:
+        wk["npose"] = args.poses
+    w = build_workload(**wk)
     pb = w["problem"]
     n = len(w["poses"])
     dev = torch.device("cuda:0")
@@ -217,7 +239,9 @@ def main():
     mixed = None if args.fp64 else measure(True)
     head = mixed or fp64
     cpu = None
-    if not args.no_cpu_baseline:
+    if args.workload == "large":
+        log("no CPU baseline for the large workload (one z plane is hours of CPU work)")
+    elif not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(w)
         except Exception as e:  # the baseline is reported, never required
@@ -228,10 +252,13 @@ def main():
            "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 contraction, f64 fit" if mixed else "f64", "data": "synthetic",
-           "config": {"workload": "receptor 3000 / ligand 1500 atoms, L=15, 51 q "
-                                  "on [0,0.5], z 25..60 step 1, 19 betas (10 deg), 64-point "
-                                  "alpha/gamma grids, 210000 poses, c1 [0.95,1.05], "
-                                  "c2 [0,4]",
+           "config": {"workload": f"receptor {wk['n_rec']} / ligand {wk['n_lig']} atoms, "
+                                  f"L={wk['L']}, {wk['qnum']} q on [0,{wk['qmax']}], z "
+                                  f"{wk['zlo']}..{wk['zhi']} step {wk['zstep']}, "
+                                  f"{wk['nbeta']} betas ({180 / (wk['nbeta'] - 1):g} deg), "
+                                  f"{wk['ngrid']}-point alpha/gamma grids, {n} poses, "
+                                  "c1 [0.95,1.05], c2 [0,4]",
+                      "name": args.workload,
                       "form": "mixed-precision" if mixed else "fp64",
                       "poses": n, "L": w["L"], "qnum": int(pb.qnum), "nbeta": w["nbeta"],
                       "ngrid": w["ngrid"], "znum": int(w["zv"].size)},

@@ -251,3 +251,46 @@ def test_atoms2spf_vs_oracle(F, L):
     scale = np.maximum(np.abs(ro).max(axis=2, keepdims=True), np.abs(io).max(axis=2, keepdims=True))
     assert np.all(np.abs(rg - ro) <= 1e-12 * scale)
     assert np.all(np.abs(ig - io) <= 1e-12 * scale)
+
+
+def test_large_complex_grid_vs_direct(F):
+    """The large single-complex grid (L = 25, 81 q on [0, 0.6], 37 betas in
+    5 degree steps, 128-point alpha/gamma grids, z in 0.5 A steps over
+    20..70 A) with small molecules: six poses of both forms against the
+    oracle's independent per-pose evaluation (rotation, translation and
+    pairing per pose; exact pi on both sides)."""
+    from libfmftsaxs_amd import synthetic as S
+    O.set_pi(O.PI_EXACT)
+    rec, lig = S.make_dimer_inputs(600, 300, 31, 32, water=True)
+    L, nb, ng = 25, 37, 128
+    q = O.mkarray(0.0, 0.6, 81)
+    zv = np.arange(20.0, 70.0 + 0.25, 0.5)
+    Ar, Ai, rmA = O.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
+    Br, Bi, rmB = O.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
+    rng = np.random.default_rng(44)
+    idx = np.sort(rng.choice(zv.size * nb * nb * ng ** 3, 6, replace=False)).astype(np.int64)
+    cross_o = []
+    for i in idx:
+        z, b1, b2, a2, g1, g2 = O.disassemble_index(int(i), nb, ng, ng)
+        cross_o.append(O.direct_pose_cross_terms(
+            Ar, Ai, Br, Bi, L, q, zv[z], b1 * np.pi / (nb - 1), 2 * np.pi * g1 / ng,
+            2 * np.pi * a2 / ng, b2 * np.pi / (nb - 1), 2 * np.pi * g2 / ng))
+    cross_o = np.array(cross_o)
+    I0 = _I(cross_o[0])
+    Iexp = I0 * (1 + 0.01 * rng.normal(size=q.size))
+    a = O.scoring_helper(q, Iexp, 0.05 * Iexp, q)
+    rm = (rmA * rec.natoms + rmB * lig.natoms) / (rec.natoms + lig.natoms)
+    P = O.Params(a, rm, Iexp[0])
+    pb = F.ScoringProblem(L, q, zv, Ar, Ai, Br, Bi, a, P.mult, Iexp[0], nbeta=nb, nalpha=ng,
+                          ngamma=ng, pi=np.pi)
+    want = [O.fit_one(c, q, P) for c in cross_o]
+    for mixed, tol in ((False, 1e-10), (True, 1e-5)):
+        print(f"large grid, L 25, 81 q, mixed={mixed}", flush=True)
+        r = F.score(pb, idx, want_cross=True, mixed=mixed)
+        relI = np.abs(_I(r["cross"]) - _I(cross_o)) / np.abs(_I(cross_o))
+        print(f"large grid mixed={mixed}: max rel I {relI.max():.2e}")
+        assert relI.max() <= tol
+        if not mixed:
+            for k, (sc, c1, c2, _, _) in enumerate(want):
+                assert abs(r["score"][k] - sc) <= 1e-7 * sc
+                assert abs(r["c1"][k] - c1) <= 1e-6 and abs(r["c2"][k] - c2) <= 1e-6
