@@ -1322,14 +1322,21 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         free(coff);
         return FSX_ERR_NOMEM;
     }
-    /* at most 4 entries per lane: more chunks until the largest fits */
-    int ncmax, ne;
+    /* more chunks until the largest fits: at most 4 entries per lane, and
+     * for FP32 tables at most 128 pairing slots (the two-poses-per-wavefront
+     * pairing); FSX_CROSS_CHUNK overrides the slot limit */
+    const int cmax = env_int("FSX_CROSS_CHUNK", mixed ? 2 * FSX_WAVE : 4 * FSX_WAVE, 64,
+                             4 * FSX_WAVE);
+    int ncmax, nsmax, ne;
     for (;;) {
         ne = entry_tables(L, C, mlo, ent);
-        ncmax = 0;
-        for (int c = 0; c < C; c++)
-            if (ent[4 * K2 + c] > ncmax) ncmax = ent[4 * K2 + c];
-        if (ncmax <= 4 * FSX_WAVE || C == N) break;
+        ncmax = nsmax = 0;
+        for (int c = 0; c < C; c++) {
+            const int nc = ent[4 * K2 + c], ns = nc - (c == 0 ? npk : 0);
+            if (nc > ncmax) ncmax = nc;
+            if (ns > nsmax) nsmax = ns;
+        }
+        if ((ncmax <= 4 * FSX_WAVE && nsmax <= cmax) || C == N) break;
         C++;
     }
     if (ncmax > 4 * FSX_WAVE) {

@@ -124,9 +124,11 @@ def _break_symmetry(pb, L, seed):
 def test_large_L_chunked_vs_oracle(F):
     """L = 25, the largest L of the workloads.  The translation blocks (6201
     entries) are too large for LDS and are read from global memory.  Physical
-    coefficients use the half entry set (351 per row, two m-chunk launches
-    that accumulate in place; both forms at their bars); coefficients without
-    the conjugate symmetry use all 676 entries (three chunks)."""
+    coefficients use the half entry set (351 per row, 338 pairing slots: two
+    m-chunk launches of at most 256 entries for FP64 tables, three of at most
+    128 slots for FP32 tables, accumulated in place; both forms at their
+    bars); coefficients without the conjugate symmetry use all 676 entries
+    (three chunks)."""
     L = 25
     zv = np.arange(30.0, 34.0)
     pb = build_problem(500, 300, L, 0.6, 11, zv, 24, seed=25)
@@ -139,7 +141,7 @@ def test_large_L_chunked_vs_oracle(F):
     assert st["cross_chunks"] == -(-st["cross_entries"] // 256)
     rel = _cmp_results(pb, r_g, r_o)
     r_m = F.score(gp, pb["index"], want_cross=True, mixed=True)
-    assert r_m["stats"]["cross_chunks"] == st["cross_chunks"]
+    assert r_m["stats"]["cross_chunks"] == (3 if st["cross_entries"] == 351 else 6)
     ok = ~np.isnan(r_o["score"])
     relI = np.abs(_I(r_m["cross"][ok]) - _I(r_o["cross"][ok])) / np.abs(_I(r_o["cross"][ok]))
     relc = np.abs(r_m["score"][ok] - r_o["score"][ok]) / r_o["score"][ok]
