@@ -49,11 +49,12 @@ WORKLOADS = {
 
 
 def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60, zstep=1.0,
-                   nbeta=19, ngrid=64, n_rec=3000, n_lig=1500, c1=(0.95, 1.05), c2=(0.0, 4.0)):
+                   nbeta=19, ngrid=64, n_rec=3000, n_lig=1500, c1=(0.95, 1.05), c2=(0.0, 4.0),
+                   water=True):
     import libfmftsaxs_amd as F
     from libfmftsaxs_amd import synthetic as S
     t0 = time.time()
-    rec, lig = S.make_dimer_inputs(n_rec, n_lig, 1001 + seed, 2001 + seed)
+    rec, lig = S.make_dimer_inputs(n_rec, n_lig, 1001 + seed, 2001 + seed, water=water)
     q = F.mkarray(0.0, qmax, qnum)
     Ar, Ai, rmA = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
     Br, Bi, rmB = F.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
@@ -74,6 +75,77 @@ def build_workload(seed=0, npose=210000, L=15, qnum=51, qmax=0.5, zlo=25, zhi=60
     return dict(problem=pb, poses=poses, q=q, zv=zv, L=L, A=(Ar, Ai), B=(Br, Bi), rm=rm,
                 a=a, mult=mult, peak=peak, exp=(q, Iexp, 0.05 * Iexp), nbeta=nbeta,
                 ngrid=ngrid, c1=c1, c2=c2)
+
+
+def run_batch(args):
+    """The batched form: NCOMPLEX synthetic complexes (receptor atoms
+    log-uniform over 500..6000, ligand 300..3000, L from {10, 15, 20}, poses
+    log-uniform over 1e4..1e6; 51 q on [0, 0.5], z 25..60 A, 10 degree betas,
+    64-point alpha/gamma grids; no water shell, to keep the set-up short),
+    scored by one fsx_score_batch call per step."""
+    import torch
+    import libfmftsaxs_amd as F
+    rng = np.random.default_rng(5000)
+    ws = []
+    for i in range(args.complexes):
+        n_rec = int(np.exp(rng.uniform(np.log(500), np.log(6000))))
+        n_lig = int(np.exp(rng.uniform(np.log(300), np.log(3000))))
+        L = int(rng.choice([10, 15, 20]))
+        npose = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        ws.append(build_workload(seed=100 + i, npose=npose, L=L, n_rec=n_rec, n_lig=n_lig,
+                                 water=False))
+        log(f"complex {i}: {n_rec} / {n_lig} atoms, L={L}, {npose} poses")
+    dev = torch.device("cuda:0")
+    pbs = [w["problem"] for w in ws]
+    ns = [len(w["poses"]) for w in ws]
+    d_poses = [torch.from_numpy(w["poses"]).to(dev) for w in ws]
+    outs = [[torch.full((n,), float("nan"), dtype=torch.float64, device=dev) for _ in range(3)]
+            for n in ns]
+    torch.cuda.synchronize()
+
+    def measure(mixed):
+        def step():
+            return F.score_batch_device(pbs, [p.data_ptr() for p in d_poses], ns,
+                                        [o[0].data_ptr() for o in outs],
+                                        [o[1].data_ptr() for o in outs],
+                                        [o[2].data_ptr() for o in outs], mixed=mixed)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        assert all(int(torch.isnan(o[0]).sum()) == 0 for o in outs)
+        last = stats[-1]
+        cf = sum(s["cross_flops"] for s in last)
+        ms_cross = sum(s["ms_cross"] for s in last)
+        ms_fit = sum(s["ms_fit"] for s in last)
+        peak = 157.3 if mixed else 78.6
+        ach = cf / (ms_cross * 1e-3) / 1e12
+        return {"value": sum(ns) / (dt / args.steps), "ms_per_step": dt * 1e3 / args.steps,
+                "ms_cross": ms_cross, "ms_fit": ms_fit,
+                "roofline": {"kernel": "k_cross", "bound": "mfma", "achieved": ach, "peak": peak,
+                             "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                             "ms_per_launch": ms_cross / len(ns), "algorithmic_flops": cf,
+                             "arith": "f32" if mixed else "f64"}}
+
+    fp64 = measure(False)
+    mixed = measure(True)
+    print(json.dumps({
+        "metric": "poses scored/sec incl. c1,c2 fit: mixed-precision vs FP64 GPU form, 1 GPU",
+        "value": mixed["value"], "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 contraction, f64 fit",
+        "data": "synthetic",
+        "config": {"workload": f"{len(ns)} complexes (receptor 500..6000, ligand 300..3000 "
+                               "atoms log-uniform, L from {10, 15, 20}, 1e4..1e6 poses "
+                               "log-uniform; 51 q on [0,0.5], z 25..60, 19 betas, 64-point "
+                               "alpha/gamma grids, no water shell) in one fsx_score_batch call",
+                   "name": "batch", "poses": sum(ns),
+                   "L": [int(w["L"]) for w in ws]},
+        "roofline": mixed["roofline"], "fp64_form": fp64,
+        "mixed_over_fp64": mixed["value"] / fp64["value"], "cpu_baseline": None}))
 
 
 def cpu_baseline(w, max_threads=16):
@@ -139,9 +211,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--poses", type=int, default=0, help="pose count (default: the workload's)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="standard",
-                    help="standard (the headline workload), large (L = 25 single complex) "
-                         "or small (the CPU-runnable case)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch"], default="standard",
+                    help="standard (the headline workload), large (L = 25 single complex), "
+                         "small (the CPU-runnable case) or batch (many complexes in one "
+                         "call)")
+    ap.add_argument("--complexes", type=int, default=24, help="complexes of the batch workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true",
                     help="time only the mixed-precision form (FP32 contraction, FP64 fit)")
@@ -169,6 +243,9 @@ def main():
     import libfmftsaxs_amd as F
     if F.device_count() < 1:
         raise SystemExit("no HIP device")
+    if args.workload == "batch":
+        run_batch(args)
+        return
     wk = dict(WORKLOADS[args.workload])
     if args.poses:
         wk["npose"] = args.poses

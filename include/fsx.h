@@ -98,6 +98,18 @@ typedef struct fsx_stats {
 int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
               double *c1, double *c2, double *cross, unsigned flags, fsx_stats *stats);
 
+/* Batched form of fsx_score: nprob complexes, each with its own problem
+ * (L, grids, q, coefficients, fit parameters) and pose set, scored in one
+ * call on one device, in order, with the device tables of each complex built
+ * in turn (no counterpart in the reference, which scores one complex per
+ * call).  poses[i], n[i], score[i], c1[i], c2[i] and, if cross is not NULL,
+ * cross[i] (may be NULL) are those of fsx_score for problem i; stats (may be
+ * NULL) receives nprob records.  Stops at the first failing problem and
+ * returns its error code. */
+int fsx_score_batch(int nprob, const fsx_problem *pbs, const int64_t *const *poses,
+                    const int *n, double *const *score, double *const *c1, double *const *c2,
+                    double *const *cross, unsigned flags, fsx_stats *stats);
+
 /* Cross-term profile of SPF coefficients and fit of one profile on the GPU
  * path's fit code compiled for the host (min_saxs.c:196-259 semantics).
  * ct: [6][qnum] (already rescaled); out: score, c1, c2, evaluations. */

@@ -83,6 +83,10 @@ def lib():
         L.fsx_score.argtypes = [C.POINTER(Problem), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_uint, C.POINTER(Stats)]
         L.fsx_score.restype = C.c_int
+        _vpp = C.POINTER(C.c_void_p)
+        L.fsx_score_batch.argtypes = [C.c_int, C.POINTER(Problem), _vpp, C.POINTER(C.c_int), _vpp,
+                                      _vpp, _vpp, _vpp, C.c_uint, C.POINTER(Stats)]
+        L.fsx_score_batch.restype = C.c_int
         L.fsx_fit_host.argtypes = [_dp, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
         L.fsx_fit_host.restype = C.c_int
         L.fsx_fit_batch.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
@@ -191,6 +195,44 @@ def score(problem, poses, want_cross=False, out=None, mixed=False):
                          None if cross is None else cross.ctypes.data,
                          FSX_MIXED_FP32 if mixed else 0, C.byref(st)))
     return {"score": sc, "c1": c1, "c2": c2, "cross": cross, "stats": st.as_dict()}
+
+
+def _batch_call(problems, poses, ns, sc, c1, c2, cross, flags):
+    k = len(problems)
+    arr = (Problem * k)(*[p.s for p in problems])
+    vp = lambda xs: (C.c_void_p * k)(*xs)
+    st = (Stats * k)()
+    _chk(lib().fsx_score_batch(k, arr, vp(poses), (C.c_int * k)(*ns), vp(sc), vp(c1), vp(c2),
+                               None if cross is None else vp(cross), flags, st))
+    return [s_.as_dict() for s_ in st]
+
+
+def score_batch(problems, poses_list, want_cross=False, mixed=False):
+    """Batched scoring (fsx_score_batch): one call for several complexes, each
+    with its own problem and int64 pose indices.  Returns one result dict per
+    complex, as score()."""
+    poses_list = [np.ascontiguousarray(p, dtype=np.int64) for p in poses_list]
+    out = []
+    for pb, p in zip(problems, poses_list):
+        r = {"score": np.full(p.size, np.nan), "c1": np.full(p.size, np.nan),
+             "c2": np.full(p.size, np.nan)}
+        if want_cross:
+            r["cross"] = np.full((p.size, 6, pb.qnum), np.nan)
+        out.append(r)
+    stats = _batch_call(problems, [p.ctypes.data for p in poses_list],
+                        [p.size for p in poses_list], [r["score"].ctypes.data for r in out],
+                        [r["c1"].ctypes.data for r in out], [r["c2"].ctypes.data for r in out],
+                        [r["cross"].ctypes.data for r in out] if want_cross else None,
+                        FSX_MIXED_FP32 if mixed else 0)
+    for r, st in zip(out, stats):
+        r["stats"] = st
+    return out
+
+
+def score_batch_device(problems, d_poses, ns, d_score, d_c1, d_c2, mixed=False):
+    """Device-resident batched form: per-complex lists of device addresses."""
+    flags = FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT | (FSX_MIXED_FP32 if mixed else 0)
+    return _batch_call(problems, d_poses, ns, d_score, d_c1, d_c2, None, flags)
 
 
 def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0, mixed=False):

@@ -93,6 +93,22 @@ int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
     return rc;
 }
 
+int fsx_score_batch(int nprob, const fsx_problem *pbs, const int64_t *const *poses,
+                    const int *n, double *const *score, double *const *c1, double *const *c2,
+                    double *const *cross, unsigned flags, fsx_stats *stats)
+{
+    if (nprob < 0 || (nprob > 0 && (!pbs || !poses || !n || !score || !c1 || !c2)))
+        return FSX_ERR_ARG;
+    pthread_mutex_lock(&g_dw_lock);
+    int rc = FSX_OK;
+    for (int i = 0; i < nprob && rc == FSX_OK; i++)
+        rc = fsx_score_locked(&pbs[i], poses[i], n[i], score[i], c1[i], c2[i],
+                              cross ? cross[i] : NULL, flags, stats ? &stats[i] : NULL,
+                              now_ms());
+    pthread_mutex_unlock(&g_dw_lock);
+    return rc;
+}
+
 static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
                             double *c1, double *c2, double *cross, unsigned flags,
                             fsx_stats *st, double t0)

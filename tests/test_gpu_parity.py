@@ -296,3 +296,22 @@ def test_large_complex_grid_vs_direct(F):
             for k, (sc, c1, c2, _, _) in enumerate(want):
                 assert abs(r["score"][k] - sc) <= 1e-7 * sc
                 assert abs(r["c1"][k] - c1) <= 1e-6 and abs(r["c2"][k] - c2) <= 1e-6
+
+
+def test_batch_equals_single_calls(F):
+    """fsx_score_batch over complexes of different L, grids and pose counts
+    gives, for each complex, exactly the results of its own fsx_score call,
+    in both forms."""
+    pbs = [build_problem(200, 150, 6, 0.3, 11, np.arange(20.0, 25.0), 40, seed=31),
+           build_problem(300, 200, 10, 0.4, 21, np.arange(22.0, 30.0), 90, seed=32,
+                         nbeta=11, na=24, ng=20),
+           build_problem(300, 200, 15, 0.5, 25, np.arange(25.0, 31.0), 60, seed=33)]
+    gps = [_gpu_problem(F, pb) for pb in pbs]
+    for mixed in (False, True):
+        print(f"batch of {len(gps)}, mixed={mixed}", flush=True)
+        rb = F.score_batch(gps, [pb["index"] for pb in pbs], want_cross=True, mixed=mixed)
+        for gp, pb, r in zip(gps, pbs, rb):
+            rs = F.score(gp, pb["index"], want_cross=True, mixed=mixed)
+            for k in ("score", "c1", "c2", "cross"):
+                assert np.array_equal(r[k], rs[k], equal_nan=True), k
+            assert r["stats"]["n_scored"] == rs["stats"]["n_scored"]
