@@ -1310,8 +1310,9 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     const int npk = half && env_int("FSX_CROSS_PACK", 1, 0, 1) ? K / 2 : 0;
     const size_t es = mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
     hipStream_t s = g_stream;
-    /* m-chunks: one launch per chunk of entries; one chunk unless a chunk
-     * would exceed 4 entries per lane (FSX_CROSS_CHUNKS overrides) */
+    /* m-chunks: one launch per chunk of entries, starting from one chunk
+     * (FSX_CROSS_CHUNKS: the smallest count to try); more below while a chunk
+     * is too large */
     int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
     const int Tn = tpk_sq(K);
     const size_t nent = 8 * (size_t)K2 + Tn + 2;
