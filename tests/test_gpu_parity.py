@@ -147,7 +147,7 @@ def test_large_L_chunked_vs_oracle(F):
     relc = np.abs(r_m["score"][ok] - r_o["score"][ok]) / r_o["score"][ok]
     print(f"L=25 ({st['cross_entries']} entries): FP64 max rel I {rel:.2e}; "
           f"mixed max rel I {relI.max():.2e}, chi {relc.max():.2e}")
-    assert relI.max() <= 1e-5 and relc.max() <= 1e-5
+    assert relI.max() <= 1e-5 and relc.max() <= 1e-4   # chi bar: see test_gpu_mixed.py
     assert np.array_equal(np.argsort(r_m["score"][ok], kind="stable"),
                           np.argsort(r_o["score"][ok], kind="stable"))
     pb2 = _break_symmetry(pb, L, 7)
