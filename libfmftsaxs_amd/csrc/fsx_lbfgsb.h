@@ -359,8 +359,16 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
 typedef struct fsx_lb {
     double ws[FSX_LB_N][FSX_LB_M], wy[FSX_LB_N][FSX_LB_M];   /* [variable][pair] */
     double sy[FSX_LB_M][FSX_LB_M], ss[FSX_LB_M][FSX_LB_M], wt[FSX_LB_M][FSX_LB_M];
-    double wn1[2 * FSX_LB_M][2 * FSX_LB_M];
-    double wn11[FSX_LB_M][FSX_LB_M], wn12[FSX_LB_M][FSX_LB_M], wn22[FSX_LB_M][FSX_LB_M];
+    /* the blocks K11, K12, K22 of the middle matrix K (wn) and WN1, in that
+     * order (FSX_LB_KN values): in the thread's own array, or, where
+     * FSX_LB_KSHARED is defined (the GPU fit kernels), at kb in LDS with a
+     * stride of FSX_LB_KSTRIDE lanes, which keeps them out of the registers
+     * that the rest of the update needs */
+#ifdef FSX_LB_KSHARED
+    double *kb;
+#else
+    double kb[3 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M];
+#endif
     double wc[2 * FSX_LB_M];     /* c of cauchy, flat: c(1..col), c(col+1..2col) */
     double z[FSX_LB_N], r[FSX_LB_N], d[FSX_LB_N], t[FSX_LB_N];
     int index[FSX_LB_N], iwhere[FSX_LB_N], indx2[FSX_LB_N];
@@ -383,7 +391,21 @@ typedef struct fsx_lb {
 #define LB_SY(i, j) w->sy[(i) - 1][(j) - 1]
 #define LB_SS(i, j) w->ss[(i) - 1][(j) - 1]
 #define LB_WT(i, j) w->wt[(i) - 1][(j) - 1]
-#define LB_WN1(i, j) w->wn1[(i) - 1][(j) - 1]
+#define FSX_LB_KN (3 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M)
+#ifdef FSX_LB_KSHARED
+#define LB_KV(w, o) ((w)->kb[(long)(o) * FSX_LB_KSTRIDE])
+#define FSX_LB_KARG , double *kbuf
+#define FSX_LB_KSET(s) ((s).w.kb = kbuf)
+#else
+#define LB_KV(w, o) ((w)->kb[o])
+#define FSX_LB_KARG
+#define FSX_LB_KSET(s) ((void)0)
+#endif
+/* 0-based K blocks, 1-based WN1 */
+#define LB_K11(a, b) LB_KV(w, (a) * FSX_LB_M + (b))
+#define LB_K12(a, b) LB_KV(w, FSX_LB_M * FSX_LB_M + (a) * FSX_LB_M + (b))
+#define LB_K22(a, b) LB_KV(w, 2 * FSX_LB_M * FSX_LB_M + (a) * FSX_LB_M + (b))
+#define LB_WN1(i, j) LB_KV(w, 3 * FSX_LB_M * FSX_LB_M + ((i) - 1) * 2 * FSX_LB_M + ((j) - 1))
 
 FSX_HD double lb_max(double a, double b) { return a >= b ? a : b; }
 FSX_HD double lb_min(double a, double b) { return a <= b ? a : b; }
@@ -476,14 +498,14 @@ FSX_HD int lb_trsl_k(const fsx_lb *w, int col, double b1[FSX_LB_M], double b2[FS
     const int m = FSX_LB_M;
     LB_UNROLL
     for (int i = 1; i <= m; ++i)
-        if (i <= col && w->wn11[i - 1][i - 1] == 0.) return i;
+        if (i <= col && LB_K11(i - 1, i - 1) == 0.) return i;
     LB_UNROLL
     for (int i = 1; i <= m; ++i)
-        if (i <= col && w->wn22[i - 1][i - 1] == 0.) return col + i;
+        if (i <= col && LB_K22(i - 1, i - 1) == 0.) return col + i;
     if (job == 1) {
         LB_UNROLL
         for (int c = 1; c <= m; ++c)
-            if (c == col) b2[c - 1] /= w->wn22[c - 1][c - 1];
+            if (c == col) b2[c - 1] /= LB_K22(c - 1, c - 1);
         /* columns 2col .. col+1 */
         LB_UNROLL
         for (int cp = m; cp >= 1; --cp) {
@@ -492,16 +514,16 @@ FSX_HD int lb_trsl_k(const fsx_lb *w, int col, double b1[FSX_LB_M], double b2[FS
                 if (tmp != 0.) {
                     LB_UNROLL
                     for (int i = 0; i < m; i++)
-                        if (i < col) b1[i] += tmp * w->wn12[i][cp - 1];
+                        if (i < col) b1[i] += tmp * LB_K12(i, cp - 1);
                     LB_UNROLL
-                    for (int i = 0; i < cp - 1; i++) b2[i] += tmp * w->wn22[i][cp - 1];
+                    for (int i = 0; i < cp - 1; i++) b2[i] += tmp * LB_K22(i, cp - 1);
                 }
                 if (cp >= 2) {
-                    b2[cp - 2] /= w->wn22[cp - 2][cp - 2];
+                    b2[cp - 2] /= LB_K22(cp - 2, cp - 2);
                 } else {
                     LB_UNROLL
                     for (int c = 1; c <= m; ++c)
-                        if (c == col) b1[c - 1] /= w->wn11[c - 1][c - 1];
+                        if (c == col) b1[c - 1] /= LB_K11(c - 1, c - 1);
                 }
             }
         }
@@ -512,21 +534,21 @@ FSX_HD int lb_trsl_k(const fsx_lb *w, int col, double b1[FSX_LB_M], double b2[FS
                 double tmp = -b1[c - 1];
                 if (tmp != 0.) {
                     LB_UNROLL
-                    for (int i = 0; i < c - 1; i++) b1[i] += tmp * w->wn11[i][c - 1];
+                    for (int i = 0; i < c - 1; i++) b1[i] += tmp * LB_K11(i, c - 1);
                 }
-                b1[c - 2] /= w->wn11[c - 2][c - 2];
+                b1[c - 2] /= LB_K11(c - 2, c - 2);
             }
         }
     } else {
-        b1[0] /= w->wn11[0][0];
+        b1[0] /= LB_K11(0, 0);
         LB_UNROLL
         for (int j = 2; j <= m; ++j) {
             if (j <= col) {
                 double dt = 0.;
                 LB_UNROLL
-                for (int i = 0; i < j - 1; i++) dt = dt + w->wn11[i][j - 1] * b1[i];
+                for (int i = 0; i < j - 1; i++) dt = dt + LB_K11(i, j - 1) * b1[i];
                 b1[j - 1] -= dt;
-                b1[j - 1] /= w->wn11[j - 1][j - 1];
+                b1[j - 1] /= LB_K11(j - 1, j - 1);
             }
         }
         LB_UNROLL
@@ -535,11 +557,11 @@ FSX_HD int lb_trsl_k(const fsx_lb *w, int col, double b1[FSX_LB_M], double b2[FS
                 double dt = 0.;
                 LB_UNROLL
                 for (int i = 0; i < m; i++)
-                    if (i < col) dt = dt + w->wn12[i][jp - 1] * b1[i];
+                    if (i < col) dt = dt + LB_K12(i, jp - 1) * b1[i];
                 LB_UNROLL
-                for (int i = 0; i < jp - 1; i++) dt = dt + w->wn22[i][jp - 1] * b2[i];
+                for (int i = 0; i < jp - 1; i++) dt = dt + LB_K22(i, jp - 1) * b2[i];
                 b2[jp - 1] -= dt;
-                b2[jp - 1] /= w->wn22[jp - 1][jp - 1];
+                b2[jp - 1] /= LB_K22(jp - 1, jp - 1);
             }
         }
     }
@@ -1005,28 +1027,39 @@ FSX_HD int lb_formk(fsx_lb *w, int nsub, int nenter, int ileave, int iupdat, int
             LB_UNROLL
             for (int jy = 1; jy <= iy; ++jy) {
                 const int js1 = m + jy;
-                w->wn11[jy - 1][iy - 1] = LB_WN1(iy, jy) / theta;
-                w->wn22[jy - 1][iy - 1] = LB_WN1(is1, js1) * theta;
+                LB_K11(jy - 1, iy - 1) = LB_WN1(iy, jy) / theta;
+                LB_K22(jy - 1, iy - 1) = LB_WN1(is1, js1) * theta;
             }
             LB_UNROLL
-            for (int jy = 1; jy <= iy - 1; ++jy) w->wn12[jy - 1][iy - 1] = -LB_WN1(is1, jy);
+            for (int jy = 1; jy <= iy - 1; ++jy) LB_K12(jy - 1, iy - 1) = -LB_WN1(is1, jy);
             LB_UNROLL
             for (int jy = iy; jy <= m; ++jy)
-                if (jy <= col) w->wn12[jy - 1][iy - 1] = LB_WN1(is1, jy);
-            w->wn11[iy - 1][iy - 1] += LB_SY(iy, iy);
+                if (jy <= col) LB_K12(jy - 1, iy - 1) = LB_WN1(is1, jy);
+            LB_K11(iy - 1, iy - 1) += LB_SY(iy, iy);
         }
     }
-    if (lb_chol(w->wn11, col)) return -1;
+    /* the Cholesky factor of K11 on a local copy, written back */
+    double k11[FSX_LB_M][FSX_LB_M];
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) k11[i][j] = LB_K11(i, j);
+    const int c11 = lb_chol(k11, col);
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) LB_K11(i, j) = k11[i][j];
+    if (c11) return -1;
     /* K12 <- L11^{-1} K12, column by column (dtrsl job 11) */
     LB_UNROLL
     for (int js = 1; js <= m; ++js) {
         if (js <= col) {
             double b[FSX_LB_M];
             LB_UNROLL
-            for (int i = 0; i < m; i++) b[i] = w->wn12[i][js - 1];
-            lb_trsl(w->wn11, col, b, 11);
+            for (int i = 0; i < m; i++) b[i] = LB_K12(i, js - 1);
+            lb_trsl(k11, col, b, 11);
             LB_UNROLL
-            for (int i = 0; i < m; i++) w->wn12[i][js - 1] = b[i];
+            for (int i = 0; i < m; i++) LB_K12(i, js - 1) = b[i];
         }
     }
     LB_UNROLL
@@ -1038,13 +1071,23 @@ FSX_HD int lb_formk(fsx_lb *w, int nsub, int nenter, int ileave, int iupdat, int
                     double dt = 0.;
                     LB_UNROLL
                     for (int i = 0; i < m; i++)
-                        if (i < col) dt = dt + w->wn12[i][is - 1] * w->wn12[i][js - 1];
-                    w->wn22[is - 1][js - 1] += dt;
+                        if (i < col) dt = dt + LB_K12(i, is - 1) * LB_K12(i, js - 1);
+                    LB_K22(is - 1, js - 1) += dt;
                 }
             }
         }
     }
-    if (lb_chol(w->wn22, col)) return -2;
+    double k22[FSX_LB_M][FSX_LB_M];
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) k22[i][j] = LB_K22(i, j);
+    const int c22 = lb_chol(k22, col);
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) LB_K22(i, j) = k22[i][j];
+    if (c22) return -2;
     return 0;
 }
 
@@ -1479,14 +1522,14 @@ FSX_HD void lb_zero(fsx_lb *w)
         LB_UNROLL
         for (int j = 0; j < FSX_LB_M; j++) {
             w->sy[i][j] = 0.0; w->ss[i][j] = 0.0; w->wt[i][j] = 0.0;
-            w->wn11[i][j] = 0.0; w->wn12[i][j] = 0.0; w->wn22[i][j] = 0.0;
+            LB_K11(i, j) = 0.0; LB_K12(i, j) = 0.0; LB_K22(i, j) = 0.0;
         }
     }
     LB_UNROLL
     for (int i = 0; i < 2 * FSX_LB_M; i++) {
         w->wc[i] = 0.0;
         LB_UNROLL
-        for (int j = 0; j < 2 * FSX_LB_M; j++) w->wn1[i][j] = 0.0;
+        for (int j = 0; j < 2 * FSX_LB_M; j++) LB_WN1(i + 1, j + 1) = 0.0;
     }
     LB_UNROLL
     for (int i = 0; i < FSX_LB_N; i++) {
@@ -1718,10 +1761,11 @@ done:
 /* One complete fit from x = (c1, c2).  Returns the final task; x is the
  * result, *fout = chi^2, *nevals = objective evaluations. */
 FSX_HD int fsx_lbfgsb_fit(const fsx_ct *ct, const fsx_fitpar *p, double *x, double *fout,
-                          int *nevals)
+                          int *nevals FSX_LB_KARG)
 {
     fsx_lbrc S;
     double k;
+    FSX_LB_KSET(S);
     fsx_lb_start(&S, p, x[0], x[1]);
     do {
         lb_eval(ct, p, S.x, &S.f, S.g, &k);

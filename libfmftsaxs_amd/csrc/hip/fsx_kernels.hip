@@ -27,6 +27,10 @@
 #include <string.h>
 
 #include "../fsx_internal.h"
+/* the fit kernels keep the L-BFGS-B blocks K11, K12, K22 and WN1 in LDS, one
+ * column of 256 lanes (a fit block) per value */
+#define FSX_LB_KSHARED
+#define FSX_LB_KSTRIDE 256
 #include "../fsx_lbfgsb.h"
 
 #define FSX_CROSS_THREADS 256
@@ -875,9 +879,9 @@ __device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitp
     fsx_lb_start(S, fp, 1.0, 0.0);
 }
 
-/* the G(q) cache of the fit kernel: Q doubles per thread in LDS while they
- * fit in FSX_FIT_GCACHE bytes per block of 256 */
-#define FSX_FIT_GCACHE (128 * 1024)
+/* LDS of one fit block (256 lanes): the K blocks, the G(q) cache (Q doubles
+ * per thread) while it fits, the moments and the q grid */
+#define FSX_FIT_LDS (160 * 1024)
 
 __global__ void __launch_bounds__(256)
 k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
@@ -886,7 +890,10 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
       int gcache)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double sG[];   /* [Q][256] when gcache, then a [6][Q] and q [Q] */
+    /* LDS: the K blocks [FSX_LB_KN][256], then G(q) [Q][256] when gcache,
+     * then a [6][Q] and q [Q] */
+    extern __shared__ double sKf[];
+    double *sG = sKf + FSX_LB_KN * FSX_LB_KSTRIDE;
     const int nvalid = *nvalid_p;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     /* the moments and the q grid, read by every lane at the same address,
@@ -907,6 +914,7 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     t.gc = gcache ? sG + threadIdx.x : NULL;
     t.gcs = blockDim.x;
     fsx_lbrc S;
+    S.w.kb = sKf + threadIdx.x;
     unsigned long long nev = 0, ncap = 0;
     int act = s < nvalid;
     int pose = act ? recs[s].x : 0;
@@ -940,6 +948,7 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
             double c1_lo, double c1_hi, double c2_lo, double c2_hi, double *out)
 {
 #pragma clang fp contract(off)
+    extern __shared__ double sKb[];   /* K blocks [FSX_LB_KN][256] */
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     fsx_ct t;
@@ -952,7 +961,7 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     fp.factr = 1e+7; fp.pgtol = 1e-5;
     double x[2] = {1.0, 0.0}, f;
     int nev;
-    fsx_lbfgsb_fit(&t, &fp, x, &f, &nev);
+    fsx_lbfgsb_fit(&t, &fp, x, &f, &nev, sKb + threadIdx.x);
     out[4L * s] = sqrt(f);
     out[4L * s + 1] = x[0];
     out[4L * s + 2] = x[1];
@@ -1554,9 +1563,12 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue: as many lanes as are resident at once
      * (the fit state lives in registers, one 256-thread block per CU) */
+    const size_t lds_k = (size_t)FSX_LB_KN * FSX_LB_KSTRIDE * sizeof(double);
+    const size_t lds_c = 7 * (size_t)Q * sizeof(double);
     const int gcache = env_int("FSX_FIT_GCACHE", 1, 0, 1) &&
-                       (size_t)Q * 256 * sizeof(double) <= FSX_FIT_GCACHE;
-    const size_t lds_fit = (gcache ? (size_t)Q * 256 * sizeof(double) : 0) + 7 * (size_t)Q * sizeof(double);
+                       lds_k + (size_t)Q * 256 * sizeof(double) + lds_c <= FSX_FIT_LDS;
+    const size_t lds_fit = lds_k + (gcache ? (size_t)Q * 256 * sizeof(double) : 0) + lds_c;
+    if (lds_fit > FSX_FIT_LDS) RCF(FSX_ERR_ARG);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, lds_fit)
                              != hipSuccess || fit_bpc < 1))
@@ -1661,7 +1673,8 @@ extern "C" int fsx_hip_fit_batch(const double *ct, int n, const double *qvals, i
         if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_Q], qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AM], a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_fit_batch, dim3(nblk(n, 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_fit_batch, dim3(nblk(n, 256)), dim3(256),
+                               (size_t)FSX_LB_KN * FSX_LB_KSTRIDE * sizeof(double), s,
                                (const double *)g_buf[B_FITCT], n, (const double *)g_buf[B_Q], Q,
                                (const double *)g_buf[B_AM], mult, c1_lo, c1_hi, c2_lo, c2_hi,
                                (double *)g_buf[B_FITOUT]);
