@@ -358,16 +358,17 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
 
 typedef struct fsx_lb {
     double ws[FSX_LB_N][FSX_LB_M], wy[FSX_LB_N][FSX_LB_M];   /* [variable][pair] */
-    double sy[FSX_LB_M][FSX_LB_M], ss[FSX_LB_M][FSX_LB_M], wt[FSX_LB_M][FSX_LB_M];
-    /* the blocks K11, K12, K22 of the middle matrix K (wn) and WN1, in that
-     * order (FSX_LB_KN values): in the thread's own array, or, where
+    double sy[FSX_LB_M][FSX_LB_M], ss[FSX_LB_M][FSX_LB_M];
+    /* the blocks K11, K12, K22 of the middle matrix K (wn), WN1 and the
+     * factor of T (wt), in that order (FSX_LB_KN values): in the thread's own
+     * array, or, where
      * FSX_LB_KSHARED is defined (the GPU fit kernels), at kb in LDS with a
      * stride of FSX_LB_KSTRIDE lanes, which keeps them out of the registers
      * that the rest of the update needs */
 #ifdef FSX_LB_KSHARED
     double *kb;
 #else
-    double kb[3 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M];
+    double kb[4 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M];
 #endif
     double wc[2 * FSX_LB_M];     /* c of cauchy, flat: c(1..col), c(col+1..2col) */
     double z[FSX_LB_N], r[FSX_LB_N], d[FSX_LB_N], t[FSX_LB_N];
@@ -390,8 +391,8 @@ typedef struct fsx_lb {
     })
 #define LB_SY(i, j) w->sy[(i) - 1][(j) - 1]
 #define LB_SS(i, j) w->ss[(i) - 1][(j) - 1]
-#define LB_WT(i, j) w->wt[(i) - 1][(j) - 1]
-#define FSX_LB_KN (3 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M)
+#define LB_WT(i, j) LB_WT0((i) - 1, (j) - 1)
+#define FSX_LB_KN (4 * FSX_LB_M * FSX_LB_M + 4 * FSX_LB_M * FSX_LB_M)
 #ifdef FSX_LB_KSHARED
 #define LB_KV(w, o) ((w)->kb[(long)(o) * FSX_LB_KSTRIDE])
 #define FSX_LB_KARG , double *kbuf
@@ -406,6 +407,7 @@ typedef struct fsx_lb {
 #define LB_K12(a, b) LB_KV(w, FSX_LB_M * FSX_LB_M + (a) * FSX_LB_M + (b))
 #define LB_K22(a, b) LB_KV(w, 2 * FSX_LB_M * FSX_LB_M + (a) * FSX_LB_M + (b))
 #define LB_WN1(i, j) LB_KV(w, 3 * FSX_LB_M * FSX_LB_M + ((i) - 1) * 2 * FSX_LB_M + ((j) - 1))
+#define LB_WT0(a, b) LB_KV(w, 7 * FSX_LB_M * FSX_LB_M + (a) * FSX_LB_M + (b))
 
 FSX_HD double lb_max(double a, double b) { return a >= b ? a : b; }
 FSX_HD double lb_min(double a, double b) { return a <= b ? a : b; }
@@ -584,12 +586,17 @@ FSX_HD int lb_bmv(fsx_lb *w, int col, const double v1[FSX_LB_M], const double v2
             p2[i - 1] = v2[i - 1] + sum;
         }
     }
-    int info = lb_trsl(w->wt, col, p2, 11);
+    double wt[FSX_LB_M][FSX_LB_M];   /* the factor of T, read once */
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) wt[i][j] = LB_WT0(i, j);
+    int info = lb_trsl(wt, col, p2, 11);
     if (info) return info;
     LB_UNROLL
     for (int i = 1; i <= FSX_LB_M; ++i)
         if (i <= col) p1[i - 1] = v1[i - 1] / sqrt(LB_SY(i, i));
-    info = lb_trsl(w->wt, col, p2, 1);
+    info = lb_trsl(wt, col, p2, 1);
     if (info) return info;
     LB_UNROLL
     for (int i = 1; i <= FSX_LB_M; ++i)
@@ -1112,7 +1119,17 @@ FSX_HD int lb_formt(fsx_lb *w, int col, double theta)
             }
         }
     }
-    return lb_chol(w->wt, col) ? -3 : 0;
+    double wt[FSX_LB_M][FSX_LB_M];   /* factored on a local copy, written back */
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) wt[i][j] = LB_WT0(i, j);
+    const int ct = lb_chol(wt, col);
+    LB_UNROLL
+    for (int i = 0; i < FSX_LB_M; i++)
+        LB_UNROLL
+        for (int j = 0; j < FSX_LB_M; j++) LB_WT0(i, j) = wt[i][j];
+    return ct ? -3 : 0;
 }
 
 /* subalgorithms.c cmprlb (constrained problem): reduced gradient of the
@@ -1521,7 +1538,7 @@ FSX_HD void lb_zero(fsx_lb *w)
     for (int i = 0; i < FSX_LB_M; i++) {
         LB_UNROLL
         for (int j = 0; j < FSX_LB_M; j++) {
-            w->sy[i][j] = 0.0; w->ss[i][j] = 0.0; w->wt[i][j] = 0.0;
+            w->sy[i][j] = 0.0; w->ss[i][j] = 0.0; LB_WT0(i, j) = 0.0;
             LB_K11(i, j) = 0.0; LB_K12(i, j) = 0.0; LB_K22(i, j) = 0.0;
         }
     }
