@@ -128,6 +128,13 @@ __global__ void k_sbessel_batch(const double *x, int n, int P, double *out)
  * S(K) - S(K - am) + (l - am)(K - am), S(n) = n(n+1)(2n+1)/6; the block has
  * S(K) entries. */
 __host__ __device__ __forceinline__ int tpk_sq(int n) { return n * (n + 1) * (2 * n + 1) / 6; }
+/* LDS bytes of the cross-term kernel's translation block (Tn values of type F),
+ * rounded up to 8 so that the pose records after it stay aligned */
+template <typename F>
+__host__ __device__ __forceinline__ size_t cross_tst_bytes(int Tn)
+{
+    return ((size_t)Tn * sizeof(F) + 7) & ~(size_t)7;
+}
 __host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
 {
     return tpk_sq(K) - tpk_sq(K - am) + (l - am) * (K - am);
@@ -508,6 +515,7 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  * over the chunk's entries e = (l, m), stored (first chunk) or added into
  * acc[(s*Q + q)*6 + k]. */
 #define FSX_CROSS_BLOCK 256
+#define FSX_CROSS_PB 64   /* poses per batch of the cross-term pose loop */
 /* FP32 tables: at most 96 VGPRs with two gamma2 rows per tile, so that five
  * blocks of four wavefronts fit a CU (with fewer registers than 96 the kernel
  * spills); with four rows the LDS admits four blocks, and 128 VGPRs; FP64
@@ -530,6 +538,14 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     E *sU = smem + 3 * G * nc;   /* [G][3][nc] */
     F *sT = reinterpret_cast<F *>(smem + 6 * G * nc);   /* T block, transposed per |m| (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
+    /* the current batch of up to FSX_CROSS_PB of the tile's poses: their
+     * (receptor row, alpha2), read from LDS in the pose loop so that a pose
+     * waits for one global round trip (its receptor row), not two, and their
+     * six sums, written to acc by the whole block after the batch, so that
+     * no store is in flight in the pose loop */
+    double *sO = reinterpret_cast<double *>(reinterpret_cast<char *>(smem + 6 * G * nc) +
+                                            (TST ? cross_tst_bytes<F>(Tn) : 0));
+    int2 *sP = reinterpret_cast<int2 *>(sO + 6 * FSX_CROSS_PB);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
     const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
@@ -618,34 +634,45 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             if (cnt == 0) continue;
             const int start = offs[key];
             const F *Tq = T + ((long)z * Q + q) * Tn;
-            /* pose loader: receptor row entries, twiddles, accumulator, row */
+            /* pose loader: receptor row entries, twiddles, row; pb >= 0: the
+             * record of pose p is sP[p - pb], else it is read from recs */
             E rr[3][J];
             F wr[J], wi[J];
-            double old = 0.0;
-            int s = 0, prow = 0;
-            auto load_pose = [&](int p) {
-                s = start + p;
+            int prow = 0;
+            auto load_pose = [&](int p, int pb) {
+                const int s = start + p;
                 prow = 0;
 #pragma unroll
                 for (int r = 1; r < G; r++) prow += p >= cum[r];
-                int4 rec = recs[s];
-                const E *R = Rc + (((long)q * nrows + rec.y) * 3) * nc;
+                int2 rec;
+                if (pb >= 0) {
+                    rec = sP[p - pb];
+                } else {
+                    const int4 r4 = recs[s];
+                    rec = make_int2(r4.y, r4.z);
+                }
+                const E *R = Rc + (((long)q * nrows + rec.x) * 3) * nc;
+                /* no branches, so that the J slots' loads are all in flight
+                 * together: a lane's slot past the last one (only the last
+                 * of the J can be) reads slot 0, and the pairing gives it a
+                 * zero twiddle */
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    int ec = sl + LP * j;
-                    if (ec < ns) {
-                        const E w = tw2[(long)rec.z * N + em[j] + L];
-                        wr[j] = w.re;
-                        wi[j] = w.im;
+                    const int ec = sl + LP * j;
+                    const E w = tw2[(long)rec.y * N + em[j] + L];
+                    wr[j] = w.re;
+                    wi[j] = w.im;
 #pragma unroll
-                        for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + ec];
-                    }
+                    for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + (ec < ns ? ec : 0)];
                 }
-                old = (!first && writer) ? acc[((long)s * Q + q) * 6 + slot] : 0.0;
             };
             /* one pose per wavefront: its first receptor row is in flight
              * while U is formed (two poses: loaded after U, registers) */
-            if (PW == 1 && wave < cnt) load_pose(wave);
+            if (PW == 1 && wave < cnt) load_pose(wave, -1);
+            if (tid < cnt && tid < FSX_CROSS_PB) {
+                const int4 r4 = recs[start + tid];
+                sP[tid] = make_int2(r4.y, r4.z);
+            }
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
@@ -748,21 +775,35 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 }
             }
             __syncthreads();
-            for (int pp = wave; PW * pp < cnt; pp += nwave) {
+            /* batches of FSX_CROSS_PB poses; the records of the first are
+             * staged with the ligand rows */
+            for (int pb = 0; pb < cnt; pb += FSX_CROSS_PB) {
+            if (pb > 0) {
+                if (tid < FSX_CROSS_PB && pb + tid < cnt) {
+                    const int4 r4 = recs[start + pb + tid];
+                    sP[tid] = make_int2(r4.y, r4.z);
+                }
+                __syncthreads();
+            }
+            const int pe = min(cnt, pb + FSX_CROSS_PB);
+            for (int pp = wave; pb + PW * pp < pe; pp += nwave) {
                 /* this lane's pose: PW consecutive poses per wavefront (a half
-                 * past the last pose repeats it and does not write) */
-                const int p = PW * pp + half;
-                const bool act = p < cnt;
-                if (PW > 1 || pp != wave) load_pose(min(p, cnt - 1));
+                 * past the batch's last pose repeats it and does not write) */
+                const int p = pb + PW * pp + half;
+                const bool act = p < pe;
+                if (PW > 1 || pb > 0 || pp != wave) load_pose(min(p, pe - 1), pb);
+                /* every load of the pose is issued before the first use */
+                __builtin_amdgcn_sched_barrier(0);
                 const E *sUp = sU + prow * 3 * nc;
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    int ec = sl + LP * j;
-                    if (ec < ns) {
+                    const bool ok = j < J - 1 || sl + LP * j < ns;
+                    const int ec = ok ? sl + LP * j : 0;
+                    {
                         E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
                         E uv = sUp[ec], ud = sUp[nc + ec], uw = sUp[2 * nc + ec];
-                        F w_r = wr[j], w_i = wi[j];
+                        const F w_r = ok ? wr[j] : (F)0, w_i = ok ? wi[j] : (F)0;
                         /* twiddled receptor factor: w * R^X */
                         F vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
                         F dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
@@ -819,10 +860,16 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 }
-                if (writer && act) {
-                    double x = 2.0 * v[0];
-                    acc[((long)s * Q + q) * 6 + slot] = first ? x : old + x;
-                }
+                if (writer && act) sO[(p - pb) * 6 + slot] = 2.0 * (double)v[0];
+            }
+            /* the batch's sums to acc (pose-major [pose][q][6]), added to
+             * those of the earlier m-chunks */
+            __syncthreads();
+            for (int i = tid; i < (pe - pb) * 6; i += FSX_CROSS_BLOCK) {
+                const int ip = i / 6;
+                double *d = acc + ((long)(start + pb + ip) * Q + q) * 6 + (i - ip * 6);
+                *d = first ? sO[i] : *d + sO[i];
+            }
             }
             __syncthreads();
         }
@@ -1191,7 +1238,9 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         else
             kern = jn <= 1 ? k_cross<1, 0, G, E, 1> : jn <= 2 ? k_cross<2, 0, G, E, 1>
                  : jn <= 3 ? k_cross<3, 0, G, E, 1> : k_cross<4, 0, G, E, 1>;
-        size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] + (tst ? (sizeof(E) / 2) * Tn : 0);
+        size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] +
+                     (tst ? cross_tst_bytes<decltype(E::re)>(Tn) : 0) +
+                     (sizeof(double) * 6 + sizeof(int2)) * FSX_CROSS_PB;
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
         long g = grid;
