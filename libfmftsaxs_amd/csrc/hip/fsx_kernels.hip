@@ -517,9 +517,9 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
 #define FSX_CROSS_BLOCK 256
 #define FSX_CROSS_PB 64   /* poses per batch of the cross-term pose loop */
 /* FP32 tables: at most 96 VGPRs with two gamma2 rows per tile, so that five
- * blocks of four wavefronts fit a CU (with fewer registers than 96 the kernel
- * spills); with four rows the LDS admits four blocks, and 128 VGPRs; FP64
- * tables: the LDS of three blocks fills a CU, no register bound */
+ * blocks of four wavefronts fit a CU; with four rows 128 VGPRs and four
+ * blocks (five blocks of 96 VGPRs are slower: 8.1 against 5.9 ms at L = 15);
+ * FP64 tables: the LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E, int PW>
 __global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? (G >= 4 ? 4 : 5) : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
@@ -541,11 +541,11 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     /* the current batch of up to FSX_CROSS_PB of the tile's poses: their
      * (receptor row, alpha2), read from LDS in the pose loop so that a pose
      * waits for one global round trip (its receptor row), not two, and their
-     * six sums, written to acc by the whole block after the batch, so that
-     * no store is in flight in the pose loop */
-    double *sO = reinterpret_cast<double *>(reinterpret_cast<char *>(smem + 6 * G * nc) +
-                                            (TST ? cross_tst_bytes<F>(Tn) : 0));
-    int2 *sP = reinterpret_cast<int2 *>(sO + 6 * FSX_CROSS_PB);
+     * six lane sums (type F), written to acc by the whole block after the
+     * batch, so that no store is in flight in the pose loop */
+    int2 *sP = reinterpret_cast<int2 *>(reinterpret_cast<char *>(smem + 6 * G * nc) +
+                                        (TST ? cross_tst_bytes<F>(Tn) : 0));
+    F *sO = reinterpret_cast<F *>(sP + FSX_CROSS_PB);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
     const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
@@ -594,11 +594,13 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
      * sl, sl + LP, ... (J per lane) */
     constexpr int LP = FSX_WAVE / PW;
     const int sl = lane % LP, half = lane / LP;
-    int em[J];
+    /* byte offset of each slot's twiddle e^(-i m alpha2) within a row of tw2 */
+    unsigned emo[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
         int ec = sl + LP * j;
-        em[j] = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : 0;
+        const int m = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : 0;
+        emo[j] = (unsigned)(m + L) * (unsigned)sizeof(E);
     }
     /* after the lane reduction: one pose: lanes 0, 8, .., 40 hold the six
      * sums; two poses: lanes 0, 4, .., 20 of each half */
@@ -621,6 +623,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
         const int b2 = bgg / ngg, g20 = (bgg - b2 * ngg) * G;
         const int nr = ng - g20 < G ? ng - g20 : G;   /* rows of this tile */
         const E *Bq = Bc + (((long)q * nrows + b2 * ng + g20) * 3) * nc;
+        const E *Rq = Rc + (long)q * nrows * 3 * nc;   /* receptor rows of this q */
         bool staged = false;
         for (int z = 0; z < zn; z++) {
             /* the tile's buckets are adjacent keys, their poses adjacent in recs */
@@ -651,19 +654,22 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     const int4 r4 = recs[s];
                     rec = make_int2(r4.y, r4.z);
                 }
-                const E *R = Rc + (((long)q * nrows + rec.x) * 3) * nc;
                 /* no branches, so that the J slots' loads are all in flight
                  * together: a lane's slot past the last one (only the last
                  * of the J can be) reads slot 0, and the pairing gives it a
                  * zero twiddle */
+                const E *R = Rq + (long)rec.x * 3 * nc + sl;
+                const unsigned twr = (unsigned)rec.y * (unsigned)(N * sizeof(E));
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    const int ec = sl + LP * j;
-                    const E w = tw2[(long)rec.y * N + em[j] + L];
+                    const bool ok = j < J - 1 || sl + LP * j < ns;
+                    const E *Rj = ok ? R + LP * j : R - sl;
+                    const E w = *reinterpret_cast<const E *>(
+                        reinterpret_cast<const char *>(tw2) + (twr + emo[j]));
                     wr[j] = w.re;
                     wi[j] = w.im;
 #pragma unroll
-                    for (int X = 0; X < 3; X++) rr[X][j] = R[X * nc + (ec < ns ? ec : 0)];
+                    for (int X = 0; X < 3; X++) rr[X][j] = Rj[X * nc];
                 }
             };
             /* one pose per wavefront: its first receptor row is in flight
@@ -860,7 +866,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 }
-                if (writer && act) sO[(p - pb) * 6 + slot] = 2.0 * (double)v[0];
+                if (writer && act) sO[(p - pb) * 6 + slot] = v[0];
             }
             /* the batch's sums to acc (pose-major [pose][q][6]), added to
              * those of the earlier m-chunks */
@@ -868,7 +874,8 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             for (int i = tid; i < (pe - pb) * 6; i += FSX_CROSS_BLOCK) {
                 const int ip = i / 6;
                 double *d = acc + ((long)(start + pb + ip) * Q + q) * 6 + (i - ip * 6);
-                *d = first ? sO[i] : *d + sO[i];
+                const double x = 2.0 * (double)sO[i];
+                *d = first ? x : *d + x;
             }
             }
             __syncthreads();
@@ -1240,7 +1247,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                  : jn <= 3 ? k_cross<3, 0, G, E, 1> : k_cross<4, 0, G, E, 1>;
         size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] +
                      (tst ? cross_tst_bytes<decltype(E::re)>(Tn) : 0) +
-                     (sizeof(double) * 6 + sizeof(int2)) * FSX_CROSS_PB;
+                     (sizeof(decltype(E::re)) * 6 + sizeof(int2)) * FSX_CROSS_PB;
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
         long g = grid;
