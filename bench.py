@@ -297,7 +297,9 @@ def main():
         achieved = fb / (avg["ms_fit"] * 1e-3) / 1e9
         roof_fit = {"kernel": "k_fit", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
                     "unit": "GB/s", "frac": achieved / 8000.0, "traffic": traffic.get("k_fit"),
-                    "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb}
+                    "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb,
+                    "evaluations_per_fit": float(np.mean([s["fit_evals"] for s in stats]))
+                    / max(1, stats[-1]["n_scored"])}
         # the longer of the two is the dominant kernel; the other is reported beside
         roof, other = ((roof_cross, roof_fit) if avg["ms_cross"] >= avg["ms_fit"]
                        else (roof_fit, roof_cross))

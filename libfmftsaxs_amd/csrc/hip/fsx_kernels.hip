@@ -911,19 +911,37 @@ __global__ void k_fit_scale(const double *acc, const double *cint, const int *nv
 
 /* The fit's profiles, (constant part + pose part) * scale, in place of the
  * pose part, and the unscaled profiles to cross_out when asked; one thread
- * per (pose slot, q, term). */
+ * per (pose slot, q): its six terms, contiguous and 16-byte aligned, are read
+ * and written as three pairs. */
 __global__ void k_fit_prep(double *acc, const double *cint, const int4 *recs,
                            const int *nvalid_p, int Q, const double *scale, double *cross_out)
 {
 #pragma clang fp contract(off)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n6 = (long)Q * 6;
-    if (i >= (long)*nvalid_p * n6) return;
-    const int s = (int)(i / n6), r = (int)(i - (long)s * n6), q = r / 6, k = r - q * 6;
-    double x = cint[k * Q + q];
-    x = x + acc[i];
-    if (cross_out) cross_out[((long)recs[s].x * 6 + k) * Q + q] = x * 1.0;
-    acc[i] = x * scale[s];
+    if (i >= (long)*nvalid_p * Q) return;
+    const int s = (int)(i / Q), q = (int)(i - (long)s * Q);
+    double2 *a2 = reinterpret_cast<double2 *>(acc + i * 6);
+    double x[6];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+        const double2 p = a2[h];
+        x[2 * h] = p.x;
+        x[2 * h + 1] = p.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double y = cint[k * Q + q];
+        y = y + x[k];
+        x[k] = y;
+    }
+    if (cross_out) {
+        const long o = (long)recs[s].x * 6 * Q + q;
+#pragma unroll
+        for (int k = 0; k < 6; k++) cross_out[o + (long)k * Q] = x[k] * 1.0;
+    }
+    const double sc = scale[s];
+#pragma unroll
+    for (int h = 0; h < 3; h++) a2[h] = make_double2(x[2 * h] * sc, x[2 * h + 1] * sc);
 }
 
 __device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitpar *fp,
@@ -1640,7 +1658,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     hipLaunchKernelGGL(k_fit_scale, dim3(nblk(n, 256)), dim3(256), 0, s,
                        (const double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
                        (const int *)(offs + nkeys), Q, A->peak, (double *)g_buf[B_FITSC]);
-    hipLaunchKernelGGL(k_fit_prep, dim3(nblk((long)n * Q * 6, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_fit_prep, dim3(nblk((long)n * Q, 256)), dim3(256), 0, s,
                        (double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
                        (const int4 *)g_buf[B_RECS], (const int *)(offs + nkeys), Q,
                        (const double *)g_buf[B_FITSC], dcross);
