@@ -128,6 +128,16 @@ __global__ void k_sbessel_batch(const double *x, int n, int P, double *out)
  * S(K) - S(K - am) + (l - am)(K - am), S(n) = n(n+1)(2n+1)/6; the block has
  * S(K) entries. */
 __host__ __device__ __forceinline__ int tpk_sq(int n) { return n * (n + 1) * (2 * n + 1) / 6; }
+/* Entry stride (in elements) of the cross-term kernel's staged ligand rows.
+ * FP32 tables keep the 3G values of an entry together; when 3G * 8 bytes is
+ * a multiple of 32 (G = 4: 96 bytes) two elements of padding make it an odd
+ * multiple of 16, so that the entries that one wavefront reads at a U step
+ * fall on twice as many LDS bank offsets.  FP64 tables are [G][3][nc]. */
+template <typename E, int G>
+__host__ __device__ constexpr int cross_sb()
+{
+    return sizeof(E) == 8 ? ((3 * G) % 4 == 0 ? 3 * G + 2 : 3 * G) : 3 * G;
+}
 /* LDS bytes of the cross-term kernel's translation block (Tn values of type F),
  * rounded up to 8 so that the pose records after it stay aligned */
 template <typename F>
@@ -531,19 +541,20 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
     extern __shared__ double smem_raw[];
     E *smem = reinterpret_cast<E *>(smem_raw);
-    /* ligand rows: FP32 tables [nc][G][3] (the 3G values of one entry
-     * together, read as 16-byte pairs), FP64 tables [G][3][nc] */
+    /* ligand rows: FP32 tables [nc][SB] (the 3G values of one entry together,
+     * read as 16-byte pairs, then cross_sb's padding), FP64 tables [G][3][nc] */
     constexpr bool BIL = sizeof(E) == 8;
     E *sB = smem;
-    E *sU = smem + 3 * G * nc;   /* [G][3][nc] */
-    F *sT = reinterpret_cast<F *>(smem + 6 * G * nc);   /* T block, transposed per |m| (TST) */
+    constexpr int SB = cross_sb<E, G>();   /* sB: [nc][SB] (BIL) or [G][3][nc] */
+    E *sU = smem + SB * nc;   /* [G][3][nc] */
+    F *sT = reinterpret_cast<F *>(sU + 3 * G * nc);   /* T block, transposed per |m| (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     /* the current batch of up to FSX_CROSS_PB of the tile's poses: their
      * (receptor row, alpha2), read from LDS in the pose loop so that a pose
      * waits for one global round trip (its receptor row), not two, and their
      * six lane sums (type F), written to acc by the whole block after the
      * batch, so that no store is in flight in the pose loop */
-    int2 *sP = reinterpret_cast<int2 *>(reinterpret_cast<char *>(smem + 6 * G * nc) +
+    int2 *sP = reinterpret_cast<int2 *>(reinterpret_cast<char *>(sU + 3 * G * nc) +
                                         (TST ? cross_tst_bytes<F>(Tn) : 0));
     F *sO = reinterpret_cast<F *>(sP + FSX_CROSS_PB);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
@@ -687,7 +698,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 auto bdst = [&](int b) {
                     if (!BIL) return b;
                     const int rY = b / nc;
-                    return (b - rY * nc) * (3 * G) + rY;
+                    return (b - rY * nc) * SB + rY;
                 };
                 for (int b0 = tid; b0 < nb3; b0 += 4 * FSX_CROSS_BLOCK) {
                     const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
@@ -723,7 +734,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                  * as i^(-l1) B, so that i^(-l) conj(T) B = tau (i^(-l1) B): a
                  * real-by-complex sum */
                 const F *Tr = (TST ? sT : Tq) + urow;
-                const E *bp = sB + ub0 * (BIL ? 3 * G : 1);
+                const E *bp = sB + ub0 * (BIL ? SB : 1);
                 F ur[G][3], ui[G][3];
 #pragma unroll
                 for (int r = 0; r < G; r++)
@@ -734,7 +745,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     const F tv = Tr[k * ustep];
                     E bv[3 * G];
 #pragma unroll
-                    for (int i = 0; i < 3 * G; i++) bv[i] = BIL ? bp[k * (3 * G) + i] : bp[i * nc + k];
+                    for (int i = 0; i < 3 * G; i++) bv[i] = BIL ? bp[k * SB + i] : bp[i * nc + k];
 #pragma unroll
                     for (int r = 0; r < G; r++)
 #pragma unroll
@@ -1263,7 +1274,7 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
         else
             kern = jn <= 1 ? k_cross<1, 0, G, E, 1> : jn <= 2 ? k_cross<2, 0, G, E, 1>
                  : jn <= 3 ? k_cross<3, 0, G, E, 1> : k_cross<4, 0, G, E, 1>;
-        size_t lds = sizeof(E) * 6 * (size_t)G * csz[c] +
+        size_t lds = sizeof(E) * (size_t)(cross_sb<E, G>() + 3 * G) * csz[c] +
                      (tst ? cross_tst_bytes<decltype(E::re)>(Tn) : 0) +
                      (sizeof(decltype(E::re)) * 6 + sizeof(int2)) * FSX_CROSS_PB;
         /* persistent-style grid: the blocks that fit the chip at once
