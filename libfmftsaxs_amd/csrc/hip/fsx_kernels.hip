@@ -152,8 +152,10 @@ __host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
 
 /* T^{|m|}_{l l1}(q z) (fftsaxs.c:182-243), computed from (min(l,l1), max(l,l1))
  * so the mirrored entries are bitwise equal, as in fftsaxs.c:213-238.  One
- * thread per (z, q, am, l, l1) of the full cube; valid entries are stored in
- * the packed layout. */
+ * thread per (z, q, am, l1, l) of the full cube (l fastest); valid entries are
+ * stored in the packed layout, per |m| block [l1][l]: element (am, l, l1) at
+ * tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am), the order in which the
+ * cross-term kernel's U step reads them (consecutive l, consecutive threads). */
 template <typename E>
 __global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, int znum, int Q, int L)
 {
@@ -164,7 +166,7 @@ __global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, 
     if (i >= (long)znum * Q * per) return;
     int r = (int)(i % per);
     long zq = i / per;
-    int l1 = r % K, l = (r / K) % K, m = r / (K * K);
+    int l = r % K, l1 = (r / K) % K, m = r / (K * K);
     if (l < m || l1 < m) return;
     int lo = l < l1 ? l : l1, hi = l < l1 ? l1 : l;
     const double *b = bes + zq * N;
@@ -182,7 +184,7 @@ __global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, 
      * the other part is an exact zero: stored is tau = i^(l - l1) T */
     const int ph = ((l - l1) % 4 + 4) % 4;
     const double tau = ph == 0 ? re : ph == 1 ? -im : ph == 2 ? -re : im;
-    T[zq * tpk_sq(K) + tpk_row(m, l, K) + (l1 - m)] = (decltype(E::re))tau;
+    T[zq * tpk_sq(K) + tpk_row(m, m, K) + (l1 - m) * (K - m) + (l - m)] = (decltype(E::re))tau;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -547,7 +549,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     E *sB = smem;
     constexpr int SB = cross_sb<E, G>();   /* sB: [nc][SB] (BIL) or [G][3][nc] */
     E *sU = smem + SB * nc;   /* [G][3][nc] */
-    F *sT = reinterpret_cast<F *>(sU + 3 * G * nc);   /* T block, transposed per |m| (TST) */
+    F *sT = reinterpret_cast<F *>(sU + 3 * G * nc);   /* T block, [l1][l] per |m| as in T (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     /* the current batch of up to FSX_CROSS_PB of the tile's poses: their
      * (receptor row, alpha2), read from LDS in the pose loop so that a pose
@@ -558,7 +560,6 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                                         (TST ? cross_tst_bytes<F>(Tn) : 0));
     F *sO = reinterpret_cast<F *>(sP + FSX_CROSS_PB);
     const int *ent_l = ent, *ent_m = ent + K2, *mstart = ent + 5 * K2;
-    const int *tmap = ent + 7 * K2;
     const int tid = threadIdx.x, lane = tid & (FSX_WAVE - 1), wave = tid / FSX_WAVE;
     const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
     /* U items (entries ordered by decreasing row length K - |m|, so that
@@ -567,7 +568,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
      * range, run by lanes 2i and 2i + 1 and summed across the pair; every
      * other entry takes one thread.  nsplit fills the block: with nc + nsplit
      * items the longest loop is about half the longest row. */
-    const int *uperm = ent + 7 * K2 + Tn + 2;
+    const int *uperm = ent + 7 * K2 + 2;
     int nlen1 = 0;   /* trailing entries of row length 1 (|m| = L): not split */
     for (int i = nc - 1; i >= 0 && nlen1 < 2; i--) {
         const int mm = ent_m[cstart + uperm[cstart + i]];
@@ -589,11 +590,12 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     }
     const bool ustore = uact && !(usplit && (tid & 1));
     const int ub0 = uact ? mstart[um + L] - cstart : 0;
-    /* T in LDS is stored per |m| block as [l1][l] (consecutive l, i.e.
-     * consecutive threads, at consecutive addresses): element (am, l, l1)
-     * at tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am) */
-    const int urow = uact ? (TST ? tpk_row(uam, uam, K) + (ul - uam) : tpk_row(uam, ul, K)) : 0;
-    const int ustep = TST ? K - uam : 1;
+    /* T (in global memory, and its LDS copy when TST) is stored per |m|
+     * block as [l1][l] (consecutive l, i.e. consecutive threads, at
+     * consecutive addresses): element (am, l, l1) at tpk_row(am, am, K) +
+     * (l1 - am) * (K - am) + (l - am) */
+    const int urow = uact ? tpk_row(uam, uam, K) + (ul - uam) : 0;
+    const int ustep = K - uam;
     /* pairing slots: with npk > 0 (conjugate-symmetric coefficients, the
      * chunk of m = 0) the m = 0 entries l = 2i and 2i + 1 share slot i, the
      * real part of the first (real for even l) and the imaginary part of the
@@ -721,11 +723,10 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     const int j1 = i1 < Tn ? i1 : b0, j2 = i2 < Tn ? i2 : b0;
                     const int j3 = i3 < Tn ? i3 : b0;
                     F t0 = Tq[b0], t1 = Tq[j1], t2 = Tq[j2], t3 = Tq[j3];
-                    int d0 = tmap[b0], d1 = tmap[j1], d2 = tmap[j2], d3 = tmap[j3];
-                    sT[d0] = t0;
-                    if (i1 < Tn) sT[d1] = t1;
-                    if (i2 < Tn) sT[d2] = t2;
-                    if (i3 < Tn) sT[d3] = t3;
+                    sT[b0] = t0;
+                    if (i1 < Tn) sT[i1] = t1;
+                    if (i2 < Tn) sT[i2] = t2;
+                    if (i3 < Tn) sT[i3] = t3;
                 }
             }
             __syncthreads();
@@ -1409,7 +1410,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
      * is too large */
     int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
     const int Tn = tpk_sq(K);
-    const size_t nent = 8 * (size_t)K2 + Tn + 2;
+    const size_t nent = 8 * (size_t)K2 + 2;
     int *ent = (int *)malloc(sizeof(int) * nent);
     long *coff = (long *)malloc(sizeof(long) * (N + 1));
     if (!ent || !coff) {
@@ -1439,18 +1440,10 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         free(coff);
         return FSX_ERR_ARG;
     }
-    /* packed T block index -> its place in the transposed LDS copy */
-    {
-        int *tmap = ent + 7 * K2, i = 0;
-        for (int am = 0; am <= L; am++)
-            for (int l = am; l <= L; l++)
-                for (int l1 = am; l1 <= L; l1++)
-                    tmap[i++] = tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am);
-    }
     /* U-phase thread -> entry of each chunk: entries by increasing |m|
      * (decreasing row length K - |m|), in entry order within one |m| */
     {
-        int *uperm = ent + 7 * K2 + Tn + 2;
+        int *uperm = ent + 7 * K2 + 2;
         const int *em = ent + K2, *csz = ent + 4 * K2, *cst = ent + 6 * K2;
         for (int c = 0; c < C; c++) {
             int t = 0;
