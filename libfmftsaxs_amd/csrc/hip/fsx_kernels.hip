@@ -564,8 +564,9 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     const int nwave = FSX_CROSS_BLOCK / FSX_WAVE;
     /* U items (entries ordered by decreasing row length K - |m|, so that
      * the lanes of a wavefront run loops of nearly equal length).  The
-     * nsplit longest entries are split into the two halves of their l1
-     * range, run by lanes 2i and 2i + 1 and summed across the pair; every
+     * nsplit longest entries are split into the even and the odd steps of
+     * their l1 range (adjacent entries, whose LDS reads fall on different
+     * banks), run by lanes 2i and 2i + 1 and summed across the pair; every
      * other entry takes one thread.  nsplit fills the block: with nc + nsplit
      * items the longest loop is about half the longest row. */
     const int *uperm = ent + 7 * K2 + 2;
@@ -578,15 +579,15 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
     const int nsplit = nc < FSX_CROSS_BLOCK ? min(FSX_CROSS_BLOCK - nc, nc - nlen1) : 0;
     const bool uact = tid < nc + nsplit;
     const bool usplit = tid < 2 * nsplit;
-    int ue = 0, ul = 0, um = 0, uam = 0, uk0 = 0, uk1 = 0;
+    int ue = 0, ul = 0, um = 0, uam = 0, uk0 = 0, uk1 = 0, ukst = 1;
     if (uact) {
         ue = uperm[cstart + (usplit ? tid >> 1 : tid - nsplit)];
         ul = ent_l[cstart + ue];
         um = ent_m[cstart + ue];
         uam = um < 0 ? -um : um;
-        const int nk = L - uam + 1, h = (nk + 1) >> 1;
-        uk0 = usplit && (tid & 1) ? h : 0;
-        uk1 = usplit && !(tid & 1) ? h : nk;
+        uk0 = usplit && (tid & 1) ? 1 : 0;
+        uk1 = L - uam + 1;
+        ukst = usplit ? 2 : 1;
     }
     const bool ustore = uact && !(usplit && (tid & 1));
     const int ub0 = uact ? mstart[um + L] - cstart : 0;
@@ -741,7 +742,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 for (int r = 0; r < G; r++)
 #pragma unroll
                     for (int Y = 0; Y < 3; Y++) ur[r][Y] = ui[r][Y] = 0;
-                for (int k = uk0; k < uk1; k++) {
+                for (int k = uk0; k < uk1; k += ukst) {
                     /* the step's LDS reads first, then its FMAs */
                     const F tv = Tr[k * ustep];
                     E bv[3 * G];
@@ -755,8 +756,8 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                             ui[r][Y] += tv * bv[r * 3 + Y].im;
                         }
                 }
-                /* split entries: the lower half (even lane) plus the upper
-                 * half (its odd neighbour), in that order */
+                /* split entries: the even steps (even lane) plus the odd
+                 * steps (its odd neighbour), in that order */
                 if (nsplit > 0) {
 #pragma unroll
                     for (int r = 0; r < G; r++)
