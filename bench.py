@@ -276,7 +276,8 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         assert int(torch.isnan(d_score).sum()) == 0
-        keys = ["ms_tables", "ms_bessel_t", "ms_rotate", "ms_group", "ms_cross", "ms_fit"]
+        keys = ["ms_tables", "ms_bessel_t", "ms_rotate", "ms_group", "ms_cross", "ms_fit_prep",
+                "ms_fit"]
         avg = {k: float(np.mean([s[k] for s in stats])) for k in keys}
         log(f"{tag} per-stage ms: " + ", ".join(f"{k[3:]}={v:.3f}" for k, v in avg.items()))
         cb = float(np.mean([s["cross_bytes"] for s in stats]))

@@ -78,7 +78,7 @@ typedef struct fsx_stats {
     double ms_rotate;    /* rotated receptor / ligand coefficients (GPU) */
     double ms_group;     /* pose bucketing by (z, beta2, gamma2) (GPU) */
     double ms_cross;     /* per-pose cross-term profiles (GPU) */
-    double ms_fit;       /* per-pose chi^2 fits (GPU) */
+    double ms_fit;       /* per-pose chi^2 fits: the fit kernel alone (GPU) */
     double ms_total;     /* whole call, wall clock */
     double cross_bytes;  /* algorithmic bytes of the cross-term launches (receptor rows,
                             ligand rows, per-pose accumulators) */
@@ -88,6 +88,7 @@ typedef struct fsx_stats {
     int cross_entries;   /* coefficient entries per table row: (L+1)^2, or
                             (L+1)(L+2)/2 when both molecules' coefficients are
                             conjugate symmetric and only m >= 0 is formed */
+    double ms_fit_prep;  /* profile rescale (peak / I(q0)) before the fits (GPU) */
 } fsx_stats;
 
 /* Score n poses.  score/c1/c2 receive one value per pose whose z slot is

@@ -40,7 +40,8 @@ class Stats(C.Structure):
                 ("ms_rotate", C.c_double), ("ms_group", C.c_double),
                 ("ms_cross", C.c_double), ("ms_fit", C.c_double), ("ms_total", C.c_double),
                 ("cross_bytes", C.c_double), ("cross_flops", C.c_double),
-                ("cross_items", C.c_long), ("cross_chunks", C.c_int), ("cross_entries", C.c_int)]
+                ("cross_items", C.c_long), ("cross_chunks", C.c_int), ("cross_entries", C.c_int),
+                ("ms_fit_prep", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -152,22 +152,25 @@ __host__ __device__ __forceinline__ int tpk_row(int am, int l, int K)
 
 /* T^{|m|}_{l l1}(q z) (fftsaxs.c:182-243), computed from (min(l,l1), max(l,l1))
  * so the mirrored entries are bitwise equal, as in fftsaxs.c:213-238.  One
- * thread per (z, q, am, l1, l) of the full cube (l fastest); valid entries are
- * stored in the packed layout, per |m| block [l1][l]: element (am, l, l1) at
- * tpk_row(am, am, K) + (l1 - am) * (K - am) + (l - am), the order in which the
- * cross-term kernel's U step reads them (consecutive l, consecutive threads). */
+ * thread per (z, q, packed entry) in the packed layout, per |m| block
+ * [l1][l]: element (am, l, l1) at tpk_row(am, am, K) + (l1 - am) * (K - am) +
+ * (l - am), the order in which the cross-term kernel's U step reads them
+ * (consecutive l, consecutive threads). */
 template <typename E>
 __global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, int znum, int Q, int L)
 {
 #pragma clang fp contract(off)
-    const int K = L + 1, N = 2 * L + 1;
+    const int K = L + 1, N = 2 * L + 1, Tn = tpk_sq(K);
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long per = (long)K * K * K;
-    if (i >= (long)znum * Q * per) return;
-    int r = (int)(i % per);
-    long zq = i / per;
-    int l = r % K, l1 = (r / K) % K, m = r / (K * K);
-    if (l < m || l1 < m) return;
+    if (i >= (long)znum * Q * Tn) return;
+    const int r = (int)(i % Tn);
+    const long zq = i / Tn;
+    /* |m| block of packed entry r: block am holds (K - am)^2 entries from
+     * tpk_row(am, am, K) */
+    int m = 0;
+    while (m < L && r >= tpk_row(m + 1, m + 1, K)) m++;
+    const int off = r - tpk_row(m, m, K), w = K - m;
+    const int l1 = m + off / w, l = m + off % w;
     int lo = l < l1 ? l : l1, hi = l < l1 ? l1 : l;
     const double *b = bes + zq * N;
     const double *d = ds + ((long)(lo * (lo + 1) + m) * K + hi) * N;
@@ -184,7 +187,7 @@ __global__ void k_tmat(decltype(E::re) *T, const double *bes, const double *ds, 
      * the other part is an exact zero: stored is tau = i^(l - l1) T */
     const int ph = ((l - l1) % 4 + 4) % 4;
     const double tau = ph == 0 ? re : ph == 1 ? -im : ph == 2 ? -re : im;
-    T[zq * tpk_sq(K) + tpk_row(m, m, K) + (l1 - m) * (K - m) + (l - m)] = (decltype(E::re))tau;
+    T[zq * Tn + r] = (decltype(E::re))tau;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -1580,14 +1583,14 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         }
     }
 
-    hipEvent_t ev[7];
-    for (int i = 0; i < 7; i++) HIPF(hipEventCreate(&ev[i]));
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; i++) HIPF(hipEventCreate(&ev[i]));
     HIPF(hipEventRecord(ev[0], s));
     /* translation tables */
     long nbes = (long)zn * Q * N;
     hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g_buf[B_BES],
                        (const double *)g_buf[B_Z], (const double *)g_buf[B_Q], zn, Q, N);
-    long nt = (long)zn * Q * K * K2;
+    long nt = (long)zn * Q * tpk_sq(K);
     if (mixed) launch_tmat<cplx32>(s, g_buf[B_T], nt, zn, Q, L);
     else launch_tmat<cplx>(s, g_buf[B_T], nt, zn, Q, L);
     HIPF(hipGetLastError());
@@ -1667,6 +1670,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                        (double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
                        (const int4 *)g_buf[B_RECS], (const int *)(offs + nkeys), Q,
                        (const double *)g_buf[B_FITSC], dcross);
+    HIPF(hipGetLastError());
+    HIPF(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
     hipLaunchKernelGGL(k_fit, dim3((unsigned)(lanes / 256)), dim3(256), lds_fit, s,
                        (const double *)g_buf[B_ACC], (const int4 *)g_buf[B_RECS],
                        (const int *)(offs + nkeys), Q, (const double *)g_buf[B_Q],
@@ -1674,7 +1679,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                        A->c2_hi, dscore, dc1, dc2, (int *)g_buf[B_NEXT],
                        (unsigned long long *)g_buf[B_TALLY], gcache);
     HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[5], s));
+    HIPF(hipEventRecord(ev[6], s));
     if (!dev_out) {
         HIPF(hipMemcpyAsync(A->score, dscore, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         HIPF(hipMemcpyAsync(A->c1, dc1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1686,18 +1691,19 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     unsigned long long tally[2] = {0, 0};
     HIPF(hipMemcpyAsync(tail, offs + nkeys, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     HIPF(hipMemcpyAsync(tally, g_buf[B_TALLY], sizeof(tally), hipMemcpyDeviceToHost, s));
-    HIPF(hipEventRecord(ev[6], s));
+    HIPF(hipEventRecord(ev[7], s));
     HIPF(hipStreamSynchronize(s));
-    float ms[6];
-    for (int i = 0; i < 6; i++) HIPF(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-    for (int i = 0; i < 7; i++) HIPF(hipEventDestroy(ev[i]));
+    float ms[7];
+    for (int i = 0; i < 7; i++) HIPF(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    for (int i = 0; i < 8; i++) HIPF(hipEventDestroy(ev[i]));
     const int nvalid = tail[0];
     const long items = (long)Q * tail[1];
     st->ms_bessel_t = ms[0];
     st->ms_rotate = ms[1];
     st->ms_group = ms[2];
     st->ms_cross = ms[3];
-    st->ms_fit = ms[4];
+    st->ms_fit_prep = ms[4];
+    st->ms_fit = ms[5];
     st->n_scored = nvalid;
     st->fit_evals = (long)tally[0];
     st->n_capped = (int)tally[1];
