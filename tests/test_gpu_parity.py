@@ -315,3 +315,37 @@ def test_batch_equals_single_calls(F):
             for k in ("score", "c1", "c2", "cross"):
                 assert np.array_equal(r[k], rs[k], equal_nan=True), k
             assert r["stats"]["n_scored"] == rs["stats"]["n_scored"]
+
+
+def test_max_order_vs_direct(F):
+    """The largest expansion order accepted, L = 40, on its native grid (41
+    betas, 81-point alpha/gamma): two poses of both forms against the
+    oracle's direct per-pose evaluation stored in tests/golden/l40_direct.npz
+    (tests/golden/make_l40_direct.py; exact pi).  The half set has 861
+    entries: FP64 tables run in chunks of at most 256, FP32 tables in chunks
+    of at most 128 pairing slots, and at this order both read the translation
+    blocks from global memory instead of LDS."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_l40_direct as G
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "l40_direct.npz"))
+    q, zv, Ar, Ai, Br, Bi, rm = G.inputs()
+    assert np.array_equal(q, d["q"]) and np.array_equal(zv, d["zvals"])
+    idx, cross_o = d["index"], d["cross"]
+    rng = np.random.default_rng(41)
+    Iexp = _I(cross_o[0]) * (1 + 0.01 * rng.normal(size=q.size))
+    a = O.scoring_helper(q, Iexp, 0.05 * Iexp, q)
+    P = O.Params(a, rm, Iexp[0])
+    pb = F.ScoringProblem(G.L, q, zv, Ar, Ai, Br, Bi, a, P.mult, Iexp[0], nbeta=G.NB,
+                          nalpha=G.NG, ngamma=G.NG, pi=np.pi)
+    for mixed, tol in ((False, 1e-10), (True, 1e-5)):
+        print(f"L 40, mixed={mixed}", flush=True)
+        r = F.score(pb, idx, want_cross=True, mixed=mixed)
+        st = r["stats"]
+        assert st["n_scored"] == idx.size and st["cross_entries"] == 41 * 42 // 2
+        assert st["cross_chunks"] >= (7 if mixed else 4)
+        relI = np.abs(_I(r["cross"]) - _I(cross_o)) / np.abs(_I(cross_o))
+        print(f"L 40 mixed={mixed}: max rel I {relI.max():.2e}, chunks {st['cross_chunks']}")
+        assert relI.max() <= tol
+        assert np.all(np.isfinite(r["score"]))
