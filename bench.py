@@ -121,11 +121,11 @@ def run_batch(args):
         cf = sum(s["cross_flops"] for s in last)
         ms_cross = sum(s["ms_cross"] for s in last)
         ms_fit = sum(s["ms_fit"] for s in last)
-        peak = 157.3 if mixed else 78.6
+        peak = 78.6   # VALU FMA rate, FP64 and non-packed FP32
         ach = cf / (ms_cross * 1e-3) / 1e12
         return {"value": sum(ns) / (dt / args.steps), "ms_per_step": dt * 1e3 / args.steps,
                 "ms_cross": ms_cross, "ms_fit": ms_fit,
-                "roofline": {"kernel": "k_cross", "bound": "mfma", "achieved": ach, "peak": peak,
+                "roofline": {"kernel": "k_cross", "bound": "valu", "achieved": ach, "peak": peak,
                              "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
                              "ms_per_launch": ms_cross / len(ns), "algorithmic_flops": cf,
                              "arith": "f32" if mixed else "f64"}}
@@ -148,30 +148,51 @@ def run_batch(args):
         "mixed_over_fp64": mixed["value"] / fp64["value"], "cpu_baseline": None}))
 
 
-def cpu_baseline(w, max_threads=16):
-    """The oracle (FP64 CPU restatement of the reference algorithm: per-slab
-    second sums, transform to the requested grid points, one L-BFGS-B fit
-    per pose, OpenMP over slabs) on a bounded sample of this workload: the
-    poses of the z = 40 A plane."""
+def _oracle_rate(w, sample, nthr):
     import oracle as O
     O.set_pi(O.PI_TRUNC)
     L, nb, ng = w["L"], w["nbeta"], w["ngrid"]
-    poses = w["poses"]
-    size5d = nb * nb * ng ** 3
-    zsel = int(np.argmin(np.abs(w["zv"] - 40.0)))
-    sample = poses[poses // size5d == zsel]
     P = O.Params(w["a"], w["rm"], w["peak"], c1=w["c1"], c2=w["c2"])
     P.s.mult = w["mult"]
-    nthr = max(1, min(max_threads, os.cpu_count() or 1))
     t = time.time()
     r = O.compute_saxs_scores(sample, w["A"][0], w["A"][1], w["B"][0], w["B"][1], P, w["q"],
                               w["zv"], L, nbeta=nb, na=ng, ng=ng, nthreads=nthr)
     dt = time.time() - t
     assert not np.any(np.isnan(r["score"]))
-    return {"value": len(sample) / dt, "unit": "poses/s", "cores": nthr, "kind": "port",
-            "sample": f"{len(sample)} poses of this workload (all of the z = 40 A plane, "
-                      f"{nb * nb} slabs), oracle with OpenMP over slabs on {nthr} threads, "
-                      f"{dt:.1f} s"}
+    return len(sample) / dt, dt
+
+
+def cpu_baseline(w, small=None, max_threads=16):
+    """The oracle (FP64 CPU restatement of the reference algorithm: per-slab
+    second sums, transform to the requested grid points, one L-BFGS-B fit
+    per pose, OpenMP over slabs) on bounded samples of the workloads, on
+    this host's cores: all cores (at most max_threads) on the poses of the
+    z = 40 A plane, one core on the part of that plane with beta_rec = 0,
+    and all cores on every pose of the small CPU-runnable workload."""
+    L, nb, ng = w["L"], w["nbeta"], w["ngrid"]
+    poses = w["poses"]
+    size5d = nb * nb * ng ** 3
+    zsel = int(np.argmin(np.abs(w["zv"] - 40.0)))
+    sample = poses[poses // size5d == zsel]
+    nthr = max(1, min(max_threads, os.cpu_count() or 1))
+    rate, dt = _oracle_rate(w, sample, nthr)
+    out = {"value": rate, "unit": "poses/s", "cores": nthr, "kind": "port",
+           "sample": f"{len(sample)} poses of this workload (all of the z = 40 A plane, "
+                     f"{nb * nb} slabs), oracle with OpenMP over slabs on {nthr} threads, "
+                     f"{dt:.1f} s", "cpu_model": cpu_model()}
+    b1 = (sample % size5d) // (nb * ng ** 3)
+    one = sample[b1 == 0]
+    rate1, dt1 = _oracle_rate(w, one, 1)
+    out["single_core"] = {"value": rate1, "unit": "poses/s", "cores": 1,
+                          "sample": f"{len(one)} poses (the z = 40 A plane, beta_rec = 0), "
+                                    f"{dt1:.1f} s"}
+    if small is not None:
+        rs, dts = _oracle_rate(small, small["poses"], nthr)
+        out["small_workload_full"] = {"value": rs, "unit": "poses/s", "cores": nthr,
+                                      "sample": f"all {len(small['poses'])} poses of the small "
+                                                f"CPU-runnable workload (L = {small['L']}), "
+                                                f"{dts:.1f} s"}
+    return out
 
 
 def w_alg(poses, L, Q, nb, ng):
@@ -192,17 +213,31 @@ def w_alg(poses, L, Q, nb, ng):
     return 24.0 * Q * C1 * zb2 + 72.0 * Q * C2 * slabs
 
 
-def load_traffic():
-    """HBM bytes per launch of the main kernels, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by what they
-    read for a plain copy of known size); {} if absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+def load_traffic(workload):
+    """HBM bytes per launch of the main kernels on this workload, from the
+    committed rocprofv3 --pmc summary of the same workload (FETCH_SIZE and
+    WRITE_SIZE scaled by what they read for a plain copy of known size); {}
+    when no summary of this workload is committed."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     try:
         with open(path) as f:
             d = json.load(f)
+        if d.get("name") != workload:
+            return {}
         return {k: v["bytes_per_launch"] for k, v in d["kernels"].items()}
     except (OSError, KeyError, ValueError):
         return {}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -260,11 +295,21 @@ def main():
     torch.cuda.synchronize()
 
     def measure(mixed):
-        """warmup + timed steps of one form; returns its record"""
+        """a cold first call, warmup and timed steps of one form; returns its
+        record"""
         def step():
             return F.score_device(pb, d_poses.data_ptr(), n, d_score.data_ptr(),
                                   d_c1.data_ptr(), d_c2.data_ptr(), mixed=mixed)
         tag = "mixed" if mixed else "fp64"
+        # cold: device buffers and host tables (Wigner-d, d-symbols) released,
+        # so the call builds, allocates and uploads them again
+        F.release()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        cold_ms = (time.perf_counter() - t0) * 1e3
+        log(f"{tag} cold first call: {cold_ms:.1f} ms")
         for i in range(args.warmup):
             st = step()
             log(f"{tag} warmup {i}: {st['ms_total']:.1f} ms")
@@ -282,14 +327,17 @@ def main():
         log(f"{tag} per-stage ms: " + ", ".join(f"{k[3:]}={v:.3f}" for k, v in avg.items()))
         cb = float(np.mean([s["cross_bytes"] for s in stats]))
         cf = float(np.mean([s["cross_flops"] for s in stats]))
-        # Roofline of the dominant kernel.  The cross-term kernel is
-        # arithmetic on operands held in LDS and registers: its roof is the
-        # peak of its arithmetic type (FP64 78.6 TFLOP/s, where the FP64 vector
-        # and MFMA peaks coincide; FP32 157.3 TFLOP/s).  The fit stream is
+        # Roofline of the two main kernels; the stage times come from HIP
+        # events the library records on the stream its kernels run on.  The
+        # cross-term kernel is vector (VALU) arithmetic on operands held in
+        # LDS and registers: its roof is the VALU peak of its type, 78.6
+        # TFLOP/s for FP64 and for FP32 as built here (packed FP32 is
+        # disabled, so FP32 issues at the FP64 rate).  The fit stream is
         # priced against HBM (its profile read once, three results written).
-        peak = 157.3 if mixed else 78.6
+        peak = 78.6
         achieved = cf / (avg["ms_cross"] * 1e-3) / 1e12
-        roof_cross = {"kernel": "k_cross", "bound": "mfma", "achieved": achieved, "peak": peak,
+        roof_cross = {"kernel": "k_cross", "bound": "valu", "achieved": achieved, "peak": peak,
+                      "peak_basis": "VALU FMA rate, FP64 and non-packed FP32",
                       "unit": "TFLOP/s", "frac": achieved / peak,
                       "traffic": traffic.get("k_cross_" + ("f32" if mixed else "f64")),
                       "ms_per_launch": avg["ms_cross"], "algorithmic_flops": cf,
@@ -305,15 +353,10 @@ def main():
         roof, other = ((roof_cross, roof_fit) if avg["ms_cross"] >= avg["ms_fit"]
                        else (roof_fit, roof_cross))
         ms = dt * 1e3 / args.steps
-        # the reference algorithm's work for the same poses (per active
-        # (z, beta2) 24 Q C1(L) flops of first sums, per active (z, beta1,
-        # beta2) slab 72 Q C2(L) flops of second sums) as an effective rate
-        roof["w_alg_flops"] = walg
-        roof["w_alg_effective_tflops"] = walg / (ms * 1e-3) / 1e12
         return {"value": n / (dt / args.steps), "ms_per_step": ms, "stage_ms": avg,
-                "roofline": roof, "roofline_other": other}
+                "cold_first_call_ms": cold_ms, "roofline": roof, "roofline_other": other}
 
-    traffic = load_traffic()
+    traffic = load_traffic(args.workload)
     walg = w_alg(w["poses"], w["L"], int(pb.qnum), w["nbeta"], w["ngrid"])
     fp64 = None if args.mixed else measure(False)
     mixed = None if args.fp64 else measure(True)
@@ -323,7 +366,8 @@ def main():
         log("no CPU baseline for the large workload (one z plane is hours of CPU work)")
     elif not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(w)
+            small = build_workload(**WORKLOADS["small"]) if args.workload == "standard" else None
+            cpu = cpu_baseline(w, small)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed:", repr(e))
     out = {"metric": "poses scored/sec incl. c1,c2 fit: mixed-precision vs FP64 GPU form, "
@@ -342,8 +386,15 @@ def main():
                       "form": "mixed-precision" if mixed else "fp64",
                       "poses": n, "L": w["L"], "qnum": int(pb.qnum), "nbeta": w["nbeta"],
                       "ngrid": w["ngrid"], "znum": int(w["zv"].size)},
-           "stage_ms": head["stage_ms"], "roofline": head["roofline"],
-           "roofline_other": head["roofline_other"],
+           "stage_ms": head["stage_ms"], "cold_first_call_ms": head["cold_first_call_ms"],
+           "roofline": head["roofline"], "roofline_other": head["roofline_other"],
+           # the reference's slab algorithm's flops for the same poses (per
+           # active (z, beta2) 24 Q C1(L) flops of first sums, per active
+           # (z, beta1, beta2) slab 72 Q C2(L) of second sums) over this
+           # step's time: a rate of the reference algorithm's work, not a
+           # roofline figure (this path runs a different, cheaper algorithm)
+           "reference_algorithm": {"w_alg_flops": walg,
+                                   "effective_tflops": walg / (head["ms_per_step"] * 1e-3) / 1e12},
            "fp64_form": fp64, "mixed_over_fp64": (mixed["value"] / fp64["value"])
            if (mixed and fp64) else None,
            "cpu_baseline": cpu}

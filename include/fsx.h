@@ -178,7 +178,9 @@ void fsx_free(void *p);
 
 int fsx_device_count(void);
 const char *fsx_error_string(int code);
-/* Release the cached device buffers of the calling process. */
+/* Release the cached device buffers and host tables (Wigner-d, d-symbols)
+ * of the calling process; the next call builds and uploads them again (a
+ * cold call).  Waits for scoring calls in flight. */
 void fsx_release(void);
 
 #ifdef __cplusplus

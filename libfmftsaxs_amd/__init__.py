@@ -156,6 +156,12 @@ def device_count():
     return lib().fsx_device_count()
 
 
+def release():
+    """fsx_release: drop the device buffers and host tables; the next call is
+    a cold call (tables built and uploaded again)."""
+    lib().fsx_release()
+
+
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 

@@ -17,6 +17,7 @@ double fsx_sbessel_host(int l, double x);
 void fsx_wigner_d(int L, double beta, double *d);   /* (L+1)(2L+1)^2, reference layout */
 double fsx_w3j(int j1, int j2, int j3, int m1, int m2, int m3);
 const double *fsx_d_symb_cached(int L);             /* (L+1)^3 (2L+1), process-wide cache */
+void fsx_d_symb_drop(void);                         /* empty that cache */
 
 /* everything the GPU pipeline needs, prepared on the host */
 typedef struct fsx_hip_args {

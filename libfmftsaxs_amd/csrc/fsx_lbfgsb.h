@@ -24,6 +24,43 @@
  * The reference loop has no iteration cap (min_saxs.c:234-248);
  * FSX_LB_MAX_EVALS bounds it here.
  */
+/*
+ * This file restates L-BFGS-B 3.0 (Ciyou Zhu, Richard Byrd, Peihuang Lu,
+ * Jorge Nocedal; revision by Jose Luis Morales and Jorge Nocedal, 2011),
+ * following the C translation vendored by libfmftsaxs in lbfgsb/src, which
+ * carries this licence:
+ *
+ *   BSD 3-clause license
+ *   Copyright (c) 2015, Stephen Becker
+ *   All rights reserved.
+ *
+ *   Redistribution and use in source and binary forms, with or without
+ *   modification, are permitted provided that the following conditions are
+ *   met:
+ *
+ *   * Redistributions of source code must retain the above copyright notice,
+ *     this list of conditions and the following disclaimer.
+ *
+ *   * Redistributions in binary form must reproduce the above copyright
+ *     notice, this list of conditions and the following disclaimer in the
+ *     documentation and/or other materials provided with the distribution.
+ *
+ *   * Neither the name of the copyright holder nor the names of its
+ *     contributors may be used to endorse or promote products derived from
+ *     this software without specific prior written permission.
+ *
+ *   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS
+ *   IS" AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED
+ *   TO, THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A
+ *   PARTICULAR PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT
+ *   HOLDER OR CONTRIBUTORS BE LIABLE FOR ANY DIRECT, INDIRECT, INCIDENTAL,
+ *   SPECIAL, EXEMPLARY, OR CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT LIMITED
+ *   TO, PROCUREMENT OF SUBSTITUTE GOODS OR SERVICES; LOSS OF USE, DATA, OR
+ *   PROFITS; OR BUSINESS INTERRUPTION) HOWEVER CAUSED AND ON ANY THEORY OF
+ *   LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT (INCLUDING
+ *   NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY OUT OF THE USE OF THIS
+ *   SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
+ */
 #ifndef FSX_LBFGSB_H
 #define FSX_LBFGSB_H
 
@@ -66,17 +103,12 @@
  * at q index i is scale * (c[k*cks + i*cqs] + v[k*vks + i*vqs]); v may be
  * NULL.  This lets the GPU read (constant part + pose part) in place and
  * apply the peak rescale on the fly without a private copy.  With c NULL,
- * v holds the final values (already summed and rescaled).  gc, when set, is
- * a per-fit cache of the factor G(q) = c1^3 exp(-mult (c1^2 - 1) q^2) at
- * gc[i*gcs]: the scale pass stores it, the objective pass of the same point
- * reads it back instead of evaluating the same expression again. */
+ * v holds the final values (already summed and rescaled). */
 typedef struct fsx_ct {
     const double *c;
     const double *v;
     long cks, cqs, vks, vqs;
     double scale;
-    double *gc;
-    long gcs;
 } fsx_ct;
 
 FSX_HD double fsx_ct_get(const fsx_ct *t, int k, int i)
@@ -90,15 +122,17 @@ FSX_HD double fsx_ct_get(const fsx_ct *t, int k, int i)
 typedef struct fsx_fitpar {
     const double *a;      /* 6*qnum moments (scoring_helper) */
     const double *q;      /* model q grid */
+    /* optional (may be NULL): rdq[i] = 1.0 / (q[i] - q[i-1]) with q[-1] =
+     * -1.0, the reciprocal the objective forms per q (min_saxs.c:60);
+     * precomputed by the same division, so the values are identical */
+    const double *rdq;
     int qnum;
     double mult;          /* (4pi/3)^1.5 rm^2 / (16 pi) */
     double lo0, hi0, lo1, hi1;
     double factr, pgtol;
 } fsx_fitpar;
 
-/* The six terms at q index i, each (c + v) * scale as fsx_ct_get.  Fetched
- * together (and one q ahead in the loops below) so that a GPU thread has
- * all its loads of a step in flight at once. */
+/* The six terms at q index i, each (c + v) * scale as fsx_ct_get. */
 FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 {
     if (!t->c) {
@@ -121,7 +155,7 @@ FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 
 /* Terms at q = i0 .. i0+nb-1 (nb <= FSX_QB) into blk: all raw loads first,
  * then (c + v) * scale per value exactly as fsx_ct_get. */
-#define FSX_QB 8
+#define FSX_QB 4
 FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
 {
     if (!t->c) {
@@ -129,6 +163,9 @@ FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
         const long vks = t->vks, vqs = t->vqs;
 #if defined(__HIP_DEVICE_COMPILE__)
         if (vks == 1 && vqs == 6) {
+            /* pose-major profile (the fit kernel's accumulator): the six
+             * terms of one q are contiguous and 16-byte aligned, so they are
+             * read as three pairs */
             const double2 *v2 = reinterpret_cast<const double2 *>(v);
 #pragma unroll
             for (int u = 0; u < FSX_QB; u++)
@@ -157,22 +194,6 @@ FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
         double vr[FSX_QB][6];
         const double *v = t->v + i0 * t->vqs;
         const long vks = t->vks, vqs = t->vqs;
-#if defined(__HIP_DEVICE_COMPILE__)
-        if (vks == 1 && vqs == 6) {
-            /* pose-major profile (the fit kernel's accumulator): the six
-             * terms of one q are contiguous and 16-byte aligned, so they are
-             * read as three pairs */
-            const double2 *v2 = reinterpret_cast<const double2 *>(v);
-#pragma unroll
-            for (int u = 0; u < FSX_QB; u++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const double2 p = u < nb ? v2[u * 3 + k] : make_double2(0.0, 0.0);
-                    vr[u][2 * k] = p.x;
-                    vr[u][2 * k + 1] = p.y;
-                }
-        } else
-#endif
 #ifdef __HIPCC__
 #pragma unroll
 #endif
@@ -208,111 +229,136 @@ FSX_HD double fsx_in(const fsx_ct *t, int i, double G, double c2)
     return FSX_IN(v, G, c2);
 }
 
-/* The q loops below fetch the profile FSX_QB q values at a time (all loads
- * of a block in flight together), then run the steps of the block in order;
- * the arithmetic and its order are those of the reference loops. */
+/* The q loops below walk the profile in blocks of FSX_QB q values, two
+ * blocks at a time: the loads of the next block are issued before the steps
+ * of the current one run, so that a GPU thread's profile reads overlap its
+ * arithmetic.  Each q step's arithmetic and the order of the sums are those
+ * of the reference loops.  BODY(cur, i) runs the step of q index i on the six
+ * terms cur[0..5]. */
+#define FSX_QB_MIN(a, b) ((a) < (b) ? (a) : (b))
+#define FSX_QLOOP(t, qnum, bA, BODY)                                                  \
+    do {                                                                              \
+        double bB_[FSX_QB][6];                                                        \
+        for (int i0_ = 0; i0_ < (qnum); i0_ += 2 * FSX_QB) {                          \
+            if (i0_ + FSX_QB < (qnum))                                                \
+                fsx_ct_block((t), i0_ + FSX_QB, FSX_QB_MIN((qnum) - i0_ - FSX_QB, FSX_QB), bB_); \
+            FSX_QLOOP_UNROLL                                                          \
+            for (int u_ = 0; u_ < FSX_QB; u_++)                                       \
+                if (i0_ + u_ < (qnum)) { BODY(bA[u_], i0_ + u_); }                    \
+            if (i0_ + 2 * FSX_QB < (qnum))                                            \
+                fsx_ct_block((t), i0_ + 2 * FSX_QB,                                   \
+                             FSX_QB_MIN((qnum) - i0_ - 2 * FSX_QB, FSX_QB), bA);      \
+            FSX_QLOOP_UNROLL                                                          \
+            for (int u_ = 0; u_ < FSX_QB; u_++)                                       \
+                if (i0_ + FSX_QB + u_ < (qnum)) { BODY(bB_[u_], i0_ + FSX_QB + u_); } \
+        }                                                                             \
+    } while (0)
+#ifdef __HIPCC__
+#define FSX_QLOOP_UNROLL _Pragma("unroll")
+#else
+#define FSX_QLOOP_UNROLL
+#endif
 
-/* min_saxs.c:261-319 */
-FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2)
+/* min_saxs.c:261-319.  bA holds the first block of the profile (q = 0 ..
+ * FSX_QB-1) on entry; it is overwritten. */
+FSX_HD double fsx_best_scale_blk(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2,
+                                 double bA[FSX_QB][6])
 {
     const double *a = p->a, *q = p->q;
     const int qnum = p->qnum;
-    double v0[6];
-    fsx_ct_get6(t, 0, v0);
     double corr = -p->mult * (c1 * c1 - 1.0);
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
-    double in_prev = FSX_IN(v0, G, c2);
+    double in_prev = FSX_IN(bA[0], G, c2);
     double c1c = c1 * c1 * c1;
     double q_prev = -1.0, up = 0.0, down = 0.0;
-    for (int i0 = 0; i0 < qnum; i0 += FSX_QB) {
-        double blk[FSX_QB][6];
-        fsx_ct_block(t, i0, qnum - i0 < FSX_QB ? qnum - i0 : FSX_QB, blk);
-#ifdef __HIPCC__
-#pragma unroll
-#endif
-        for (int u = 0; u < FSX_QB; u++) {
-            const int i = i0 + u;
-            if (i < qnum) {
-                const double *cur = blk[u];
-                double qc = q[i];
-                G = c1c * exp(corr * qc * qc);
-                if (t->gc) t->gc[i * t->gcs] = G;
-                double in = FSX_IN(cur, G, c2);
-                double tan_ = (in - in_prev) / (qc - q_prev);
-                double buf = in - tan_ * qc;
-                up += buf * a[i * 6 + 1] + tan_ * a[i * 6 + 2];
-                down += buf * buf * a[i * 6 + 3] + 2.0 * tan_ * buf * a[i * 6 + 4]
-                      + tan_ * tan_ * a[i * 6 + 5];
-                in_prev = in;
-                q_prev = qc;
-            }
-        }
-    }
+#define FSX_BS_STEP(cur, i)                                                           \
+    do {                                                                              \
+        double qc = q[i];                                                             \
+        G = c1c * exp(corr * qc * qc);                                                \
+        double in = FSX_IN(cur, G, c2);                                               \
+        double tan_ = (in - in_prev) / (qc - q_prev);                                 \
+        double buf = in - tan_ * qc;                                                  \
+        up += buf * a[(i) * 6 + 1] + tan_ * a[(i) * 6 + 2];                           \
+        down += buf * buf * a[(i) * 6 + 3] + 2.0 * tan_ * buf * a[(i) * 6 + 4]        \
+              + tan_ * tan_ * a[(i) * 6 + 5];                                         \
+        in_prev = in;                                                                 \
+        q_prev = qc;                                                                  \
+    } while (0)
+    FSX_QLOOP(t, qnum, bA, FSX_BS_STEP);
+#undef FSX_BS_STEP
     return up / down;
 }
 
-/* min_saxs.c:3-105 */
-FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
-                          const double *x, double *g, double *fval)
+FSX_HD double fsx_best_scale(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2)
 {
-    const double *a = p->a, *q = p->q;
+    double bA[FSX_QB][6];
+    fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
+    return fsx_best_scale_blk(t, p, c1, c2, bA);
+}
+
+/* min_saxs.c:3-105.  bA holds the first block of the profile on entry; it is
+ * overwritten. */
+FSX_HD void fsx_objective_blk(const fsx_ct *t, const fsx_fitpar *p, double k, const double *x,
+                              double *g, double *fval, double bA[FSX_QB][6])
+{
+    const double *a = p->a, *q = p->q, *rdq = p->rdq;
     const int qnum = p->qnum;
     double v0[6];
-    fsx_ct_get6(t, 0, v0);
+    for (int j = 0; j < 6; j++) v0[j] = bA[0][j];
     double c1 = x[0], c2 = x[1];
     double g0 = 0.0, g1 = 0.0, mult = p->mult;
     double corr = -mult * (c1 * c1 - 1.0);
-    double G = t->gc ? t->gc[0] : c1 * c1 * c1 * exp(corr * q[0] * q[0]);
+    double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
     double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * q[0] * q[0]);
     double in_prev = FSX_IN(v0, G, c2);
     double d1p = -Gd * v0[1] + 2.0 * G * Gd * v0[3] - Gd * c2 * v0[4];
     double d2p = v0[2] - G * v0[4] + 2.0 * c2 * v0[5];
     double q_prev = -1.0, score = 0.0, c1c = c1 * c1 * c1;
-    for (int i0 = 0; i0 < qnum; i0 += FSX_QB) {
-        double blk[FSX_QB][6];
-        fsx_ct_block(t, i0, qnum - i0 < FSX_QB ? qnum - i0 : FSX_QB, blk);
-#ifdef __HIPCC__
-#pragma unroll
-#endif
-        for (int u = 0; u < FSX_QB; u++) {
-            const int i = i0 + u;
-            if (i < qnum) {
-                const double *cur = blk[u];
-                double qc = q[i];
-                G = t->gc ? t->gc[i * t->gcs] : c1c * exp(corr * qc * qc);
-                Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
-                double vd = cur[1], vw = cur[2], dd = cur[3];
-                double dw = cur[4], ww = cur[5];
-                double in = cur[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww;
-                double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);
-                double d2 = vw - G * dw + 2.0 * c2 * ww;
-                double buf = 1.0 / (qc - q_prev);
-                double tan_ = (in - in_prev) * buf;
-                double t1 = (d1 - d1p) * buf;
-                double t2 = (d2 - d2p) * buf;
-                const double *ai = a + i * 6;
-                g0 += 2.0 * k * (-(d1 - t1 * qc) * ai[1] - t1 * ai[2]
-                                 + k * ((in - tan_ * qc) * (d1 - t1 * qc) * ai[3]
-                                        + (in * t1 + d1 * tan_ - 2.0 * tan_ * t1 * qc) * ai[4]
-                                        + tan_ * t1 * ai[5]));
-                g1 += 2.0 * k * (-(d2 - t2 * qc) * ai[1] - t2 * ai[2]
-                                 + k * ((in - tan_ * qc) * (d2 - t2 * qc) * ai[3]
-                                        + (in * t2 + d2 * tan_ - 2.0 * tan_ * t2 * qc) * ai[4]
-                                        + tan_ * t2 * ai[5]));
-                buf = in - tan_ * qc;
-                score += ai[0] + k * (-2.0 * buf * ai[1] - 2.0 * tan_ * ai[2]
-                                      + k * (buf * buf * ai[3] + 2.0 * buf * tan_ * ai[4]
-                                             + tan_ * tan_ * ai[5]));
-                in_prev = in;
-                d1p = d1;
-                d2p = d2;
-                q_prev = qc;
-            }
-        }
-    }
+#define FSX_OBJ_STEP(cur, i)                                                          \
+    do {                                                                              \
+        double qc = q[i];                                                             \
+        G = c1c * exp(corr * qc * qc);                                                \
+        Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);                              \
+        double vd = (cur)[1], vw = (cur)[2], dd = (cur)[3];                           \
+        double dw = (cur)[4], ww = (cur)[5];                                          \
+        double in = (cur)[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww; \
+        double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);                              \
+        double d2 = vw - G * dw + 2.0 * c2 * ww;                                      \
+        double buf = rdq ? rdq[i] : 1.0 / (qc - q_prev);                              \
+        double tan_ = (in - in_prev) * buf;                                           \
+        double t1 = (d1 - d1p) * buf;                                                 \
+        double t2 = (d2 - d2p) * buf;                                                 \
+        const double *ai = a + (i) * 6;                                               \
+        g0 += 2.0 * k * (-(d1 - t1 * qc) * ai[1] - t1 * ai[2]                         \
+                         + k * ((in - tan_ * qc) * (d1 - t1 * qc) * ai[3]             \
+                                + (in * t1 + d1 * tan_ - 2.0 * tan_ * t1 * qc) * ai[4] \
+                                + tan_ * t1 * ai[5]));                                \
+        g1 += 2.0 * k * (-(d2 - t2 * qc) * ai[1] - t2 * ai[2]                         \
+                         + k * ((in - tan_ * qc) * (d2 - t2 * qc) * ai[3]             \
+                                + (in * t2 + d2 * tan_ - 2.0 * tan_ * t2 * qc) * ai[4] \
+                                + tan_ * t2 * ai[5]));                                \
+        buf = in - tan_ * qc;                                                         \
+        score += ai[0] + k * (-2.0 * buf * ai[1] - 2.0 * tan_ * ai[2]                 \
+                              + k * (buf * buf * ai[3] + 2.0 * buf * tan_ * ai[4]     \
+                                     + tan_ * tan_ * ai[5]));                         \
+        in_prev = in;                                                                 \
+        d1p = d1;                                                                     \
+        d2p = d2;                                                                     \
+        q_prev = qc;                                                                  \
+    } while (0)
+    FSX_QLOOP(t, qnum, bA, FSX_OBJ_STEP);
+#undef FSX_OBJ_STEP
     g[0] = g0;
     g[1] = g1;
     *fval = score;
+}
+
+FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
+                          const double *x, double *g, double *fval)
+{
+    double bA[FSX_QB][6];
+    fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
+    fsx_objective_blk(t, p, k, x, g, fval, bA);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1560,9 +1606,12 @@ FSX_HD void lb_zero(fsx_lb *w)
 FSX_HD void lb_eval(const fsx_ct *t, const fsx_fitpar *p, const double *x, double *f,
                     double *g, double *kscale)
 {
-    double k = fsx_best_scale(t, p, x[0], x[1]);
+    double bA[FSX_QB][6];
+    fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
+    double k = fsx_best_scale_blk(t, p, x[0], x[1], bA);
     *kscale = k;
-    fsx_objective(t, p, k, x, g, f);
+    fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
+    fsx_objective_blk(t, p, k, x, g, f, bA);
 }
 
 /* ---------------------------------------------------------------------- */

@@ -27,6 +27,7 @@
 #include <string.h>
 
 #include "../fsx_internal.h"
+#include "../fsx_pose.h"
 /* the fit kernels keep the L-BFGS-B blocks K11, K12, K22 and WN1 in LDS, one
  * column of 256 lanes (a fit block) per value */
 #define FSX_LB_KSHARED
@@ -428,16 +429,6 @@ k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int n
 }
 
 /* ---- pose bucketing --------------------------------------------------- */
-__device__ __forceinline__ void decode_pose(long long idx, int nbeta, int na, int ng, int *o)
-{
-    o[5] = (int)(idx % ng); idx /= ng;
-    o[4] = (int)(idx % ng); idx /= ng;
-    o[3] = (int)(idx % na); idx /= na;
-    o[2] = (int)(idx % nbeta); idx /= nbeta;
-    o[1] = (int)(idx % nbeta);
-    o[0] = (int)(idx / nbeta);
-}
-
 __global__ void k_fill_int(int *p, long n, int v)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,20 +441,11 @@ __global__ void k_count(const long long *poses, int n, int znum, int nbeta, int 
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    long long idx = poses[i];
-    int key = -1;
-    int2 inf = make_int2(0, 0);
-    if (idx >= 0) {
-        int o[6];
-        decode_pose(idx, nbeta, na, ng, o);
-        if (o[0] < znum) {
-            key = (o[0] * nbeta + o[2]) * ng + o[5];
-            inf = make_int2(o[1] * ng + o[4], o[3]);
-            atomicAdd(&counts[key], 1);
-        }
-    }
+    int row, alpha;
+    const int key = fsx_pose_key(poses[i], znum, nbeta, na, ng, &row, &alpha);
+    if (key >= 0) atomicAdd(&counts[key], 1);
     keys[i] = key;
-    info[i] = inf;
+    info[i] = make_int2(row, alpha);
 }
 
 /* exclusive scan of counts[0..nk) into offs[0..nk]; offs[nk+1] = number of
@@ -967,31 +949,42 @@ __device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitp
     fsx_lb_start(S, fp, 1.0, 0.0);
 }
 
-/* LDS of one fit block (256 lanes): the K blocks, the G(q) cache (Q doubles
- * per thread) while it fits, the moments and the q grid */
+/* LDS of one fit block (256 lanes): the L-BFGS-B K blocks, then (when they
+ * fit, aq_lds) the moments, the q grid and the reciprocals 1/(q_i - q_{i-1}) */
 #define FSX_FIT_LDS (160 * 1024)
+static inline size_t fit_lds_k(void) { return (size_t)FSX_LB_KN * FSX_LB_KSTRIDE * sizeof(double); }
+static inline int fit_aq_lds(int Q) { return fit_lds_k() + 8 * (size_t)Q * sizeof(double) <= FSX_FIT_LDS; }
+static inline size_t fit_lds(int Q) { return fit_lds_k() + (fit_aq_lds(Q) ? 8 * (size_t)Q * sizeof(double) : 0); }
 
+template <int AQ_LDS>
 __global__ void __launch_bounds__(256)
 k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
-      double *score, double *c1o, double *c2o, int *next, unsigned long long *tally,
-      int gcache)
+      double *score, double *c1o, double *c2o, int *next, unsigned long long *tally)
 {
 #pragma clang fp contract(off)
-    /* LDS: the K blocks [FSX_LB_KN][256], then G(q) [Q][256] when gcache,
-     * then a [6][Q] and q [Q] */
+    /* LDS: the K blocks [FSX_LB_KN][256], then a [6][Q], q [Q] and
+     * 1/(q_i - q_{i-1}) [Q] when aq_lds (read by every lane at the same
+     * address); otherwise a and q are read from global memory and the
+     * reciprocals formed per step */
     extern __shared__ double sKf[];
-    double *sG = sKf + FSX_LB_KN * FSX_LB_KSTRIDE;
     const int nvalid = *nvalid_p;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    /* the moments and the q grid, read by every lane at the same address,
-     * from LDS */
-    double *sA = sG + (gcache ? (long)Q * 256 : 0), *sQ = sA + 6 * Q;
-    for (int i = threadIdx.x; i < 6 * Q; i += blockDim.x) sA[i] = a[i];
-    for (int i = threadIdx.x; i < Q; i += blockDim.x) sQ[i] = qvals[i];
-    __syncthreads();
     fsx_fitpar fp;
-    fp.a = sA; fp.q = sQ; fp.qnum = Q; fp.mult = mult;
+    if (AQ_LDS) {
+        double *sA = sKf + FSX_LB_KN * FSX_LB_KSTRIDE, *sQ = sA + 6 * Q, *sR = sQ + Q;
+        for (int i = threadIdx.x; i < 6 * Q; i += blockDim.x) sA[i] = a[i];
+        for (int i = threadIdx.x; i < Q; i += blockDim.x) {
+            const double qc = qvals[i];
+            sQ[i] = qc;
+            sR[i] = 1.0 / (qc - (i > 0 ? qvals[i - 1] : -1.0));
+        }
+        __syncthreads();
+        fp.a = sA; fp.q = sQ; fp.rdq = sR;
+    } else {
+        fp.a = a; fp.q = qvals; fp.rdq = NULL;
+    }
+    fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
     /* profiles already rescaled in place (k_fit_prep) */
@@ -999,8 +992,6 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     t.c = NULL; t.cks = 0; t.cqs = 0;
     t.v = acc; t.vks = 1; t.vqs = 6;
     t.scale = 1.0;
-    t.gc = gcache ? sG + threadIdx.x : NULL;
-    t.gcs = blockDim.x;
     fsx_lbrc S;
     S.w.kb = sKf + threadIdx.x;
     unsigned long long nev = 0, ncap = 0;
@@ -1042,9 +1033,8 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     fsx_ct t;
     t.c = ct + (long)s * 6 * Q; t.cks = Q; t.cqs = 1;
     t.v = NULL; t.vks = 0; t.vqs = 0; t.scale = 1.0;
-    t.gc = NULL; t.gcs = 0;
     fsx_fitpar fp;
-    fp.a = a; fp.q = qvals; fp.qnum = Q; fp.mult = mult;
+    fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
     double x[2] = {1.0, 0.0}, f;
@@ -1359,7 +1349,7 @@ static int entry_tables(int L, int C, int mlo, int *ent)
     return ne;
 }
 
-/* A^X_{l,-m}(q) = (-1)^(l+m) conj(A^X_{lm}(q)) for m >= 1, to 1e-13 of the
+/* A^X_{l,-m}(q) = (-1)^(l+m) conj(A^X_{lm}(q)) for m >= 0, to 1e-13 of the
  * largest coefficient of each (X, q): the amplitude of a real density.
  * Then the terms of m and -m of every cross-term sum are complex
  * conjugates of each other, and the sums run over m >= 0 with the m > 0
@@ -1376,6 +1366,12 @@ static int spf_conj_symmetric(const double *re, const double *im, int Q, int L)
             mx = b > mx ? b : mx;
         }
         const double tol = 1e-13 * mx;
+        /* m = 0: A_l0 = (-1)^l conj(A_l0), i.e. real for even l and imaginary
+         * for odd l (the packed m = 0 slots keep only that part of i^l A_l0) */
+        for (int l = 0; l <= L; l++) {
+            const int e0 = l * l + l;
+            if (!(fabs((l & 1) ? r[e0] : i[e0]) <= tol)) return 0;
+        }
         for (int l = 1; l <= L; l++)
             for (int m = 1; m <= l; m++) {
                 const int ep = l * l + l + m, en = l * l + l - m;
@@ -1485,6 +1481,8 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     coff[0] = 0;
     for (int c = 0; c < C; c++) coff[c + 1] = coff[c] + (long)Q * nrows * 3 * csz[c];
     int rc = FSX_OK;
+    hipEvent_t ev[8];
+    int nev_made = 0;   /* events created so far (destroyed on every exit) */
 #define RCF(x)                             \
     do {                                   \
         rc = (x);                          \
@@ -1583,8 +1581,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         }
     }
 
-    hipEvent_t ev[8];
-    for (int i = 0; i < 8; i++) HIPF(hipEventCreate(&ev[i]));
+    for (; nev_made < 8; nev_made++) HIPF(hipEventCreate(&ev[nev_made]));
     HIPF(hipEventRecord(ev[0], s));
     /* translation tables */
     long nbes = (long)zn * Q * N;
@@ -1645,14 +1642,13 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipEventRecord(ev[4], s));
     /* per-pose fits over a pose queue: as many lanes as are resident at once
      * (the fit state lives in registers, one 256-thread block per CU) */
-    const size_t lds_k = (size_t)FSX_LB_KN * FSX_LB_KSTRIDE * sizeof(double);
-    const size_t lds_c = 7 * (size_t)Q * sizeof(double);
-    const int gcache = env_int("FSX_FIT_GCACHE", 1, 0, 1) &&
-                       lds_k + (size_t)Q * 256 * sizeof(double) + lds_c <= FSX_FIT_LDS;
-    const size_t lds_fit = lds_k + (gcache ? (size_t)Q * 256 * sizeof(double) : 0) + lds_c;
-    if (lds_fit > FSX_FIT_LDS) RCF(FSX_ERR_ARG);
+    const int aq_lds = fit_aq_lds(Q);
+    const size_t lds_fit = fit_lds(Q);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
-    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, k_fit, 256, lds_fit)
+    void (*kfit)(const double *, const int4 *, const int *, int, const double *, const double *,
+                 double, double, double, double, double, double *, double *, double *, int *,
+                 unsigned long long *) = aq_lds ? k_fit<1> : k_fit<0>;
+    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, kfit, 256, lds_fit)
                              != hipSuccess || fit_bpc < 1))
         fit_bpc = 1;
     long lanes = (long)g_ncu * 256 * fit_bpc;
@@ -1672,12 +1668,12 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
                        (const double *)g_buf[B_FITSC], dcross);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
-    hipLaunchKernelGGL(k_fit, dim3((unsigned)(lanes / 256)), dim3(256), lds_fit, s,
+    hipLaunchKernelGGL(kfit, dim3((unsigned)(lanes / 256)), dim3(256), lds_fit, s,
                        (const double *)g_buf[B_ACC], (const int4 *)g_buf[B_RECS],
                        (const int *)(offs + nkeys), Q, (const double *)g_buf[B_Q],
                        (const double *)g_buf[B_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo,
                        A->c2_hi, dscore, dc1, dc2, (int *)g_buf[B_NEXT],
-                       (unsigned long long *)g_buf[B_TALLY], gcache);
+                       (unsigned long long *)g_buf[B_TALLY]);
     HIPF(hipGetLastError());
     HIPF(hipEventRecord(ev[6], s));
     if (!dev_out) {
@@ -1695,7 +1691,6 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     HIPF(hipStreamSynchronize(s));
     float ms[7];
     for (int i = 0; i < 7; i++) HIPF(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-    for (int i = 0; i < 8; i++) HIPF(hipEventDestroy(ev[i]));
     const int nvalid = tail[0];
     const long items = (long)Q * tail[1];
     st->ms_bessel_t = ms[0];
@@ -1725,6 +1720,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     st->cross_entries = ne;
     }
 out:
+    for (int i = 0; i < nev_made; i++) (void)hipEventDestroy(ev[i]);
     free(ent);
     free(coff);
     free(tw2);

@@ -336,6 +336,7 @@ static fsx_fitpar fitpar_of(const struct sxs_profile *pr, const struct sxs_opt_p
     fsx_fitpar fp;
     fp.a = p->a;
     fp.q = pr->qvals;
+    fp.rdq = NULL;
     fp.qnum = pr->qnum;
     fp.mult = p->mult;
     fp.lo0 = 0.96; fp.hi0 = 1.04; fp.lo1 = -2.00; fp.hi1 = 4.00;   /* define.h:28-31 */
@@ -348,7 +349,6 @@ static fsx_ct ct_packed(const double *c, int qnum)
 {
     fsx_ct t;
     t.c = c; t.v = NULL; t.cks = qnum; t.cqs = 1; t.vks = 0; t.vqs = 0; t.scale = 1.0;
-    t.gc = NULL; t.gcs = 0;
     return t;
 }
 
@@ -462,7 +462,7 @@ int fsx_fit_host(const double *ct, const double *qvals, int qnum, const double *
     if (!ct || !qvals || !a || !out4 || qnum < 2) return FSX_ERR_ARG;
     fsx_ct t = ct_packed(ct, qnum);
     fsx_fitpar fp;
-    fp.a = a; fp.q = qvals; fp.qnum = qnum; fp.mult = mult;
+    fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = qnum; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
     double x[2] = {1.0, 0.0}, f;

@@ -211,3 +211,13 @@ double *sxs_mkarray(double begin, double end, int qnum)
     for (int i = 1; i < qnum; i++) q[i] = q[i - 1] + step;
     return q;
 }
+
+void fsx_d_symb_drop(void)
+{
+    pthread_mutex_lock(&g_lock);
+    for (int L = 0; L <= FSX_MAX_L; L++) {
+        free(g_dsymb[L]);
+        g_dsymb[L] = NULL;
+    }
+    pthread_mutex_unlock(&g_lock);
+}

@@ -93,9 +93,8 @@ int fsx_score_naive(int nrec, int nlig, const double *xyz, const double *vac,
         /* the fit of sxs_lbfgs_fitting (min_saxs.c:196-259), no peak rescale */
         fsx_ct t;
         t.c = ct; t.v = NULL; t.cks = qnum; t.cqs = 1; t.vks = 0; t.vqs = 0; t.scale = 1.0;
-        t.gc = NULL; t.gcs = 0;
         fsx_fitpar fp;
-        fp.a = a; fp.q = qvals; fp.qnum = qnum; fp.mult = mult;
+        fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = qnum; fp.mult = mult;
         fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
         fp.factr = 1e+7; fp.pgtol = 1e-5;
         double xs[2] = {1.0, 0.0}, f;

@@ -40,7 +40,6 @@ const char *fsx_error_string(int code)
 }
 
 int fsx_device_count(void) { return fsx_hip_device_count(); }
-void fsx_release(void) { fsx_hip_release(); }
 
 /* fftsaxs.c:27-50: sum_lm Re(A_X conj A_Y) + Re(B_X conj B_Y) per q */
 static void const_terms(const fsx_problem *pb, double *cint)
@@ -82,6 +81,19 @@ static unsigned long long g_dw_serial;
 static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
                             double *c1, double *c2, double *cross, unsigned flags,
                             fsx_stats *st, double t0);
+
+/* Device buffers and the host tables (Wigner-d of the last grid, d-symbols);
+ * the next call builds and uploads them again.  Waits for calls in flight. */
+void fsx_release(void)
+{
+    pthread_mutex_lock(&g_dw_lock);
+    free(g_dw.dW);
+    g_dw.dW = NULL;
+    g_dw.key = 0;
+    fsx_d_symb_drop();
+    fsx_hip_release();
+    pthread_mutex_unlock(&g_dw_lock);
+}
 
 int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
               double *c1, double *c2, double *cross, unsigned flags, fsx_stats *st)

@@ -94,6 +94,21 @@ def test_native_grid_vs_oracle(F, L, Q, qmax, nat):
     print(f"L={L}: max rel I diff {rel:.2e}; stats {r_g['stats']}")
 
 
+def test_many_q_vs_oracle(F):
+    """400 q points: the moments, the q grid and the reciprocals of the
+    q steps no longer fit the fit kernel's LDS beside the L-BFGS-B blocks
+    (at most 256 q do), so the fits read them from global memory; the
+    reference has no limit on the number of q points."""
+    L = 6
+    zv = np.arange(20.0, 26.0)
+    pb = build_problem(200, 150, L, 0.5, 400, zv, 40, seed=14)
+    r_o = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                                pb["params"], pb["qvals"], zv, L, want_cross=True)
+    r_g = F.score(_gpu_problem(F, pb), pb["index"], want_cross=True)
+    rel = _cmp_results(pb, r_g, r_o)
+    print(f"400 q: max rel I diff {rel:.2e}")
+
+
 def test_extended_grid_vs_oracle(F):
     """nbeta = 19 (10 deg), 64-point alpha/gamma grids: beyond the reference's
     grid; the oracle evaluates the same generalised transform directly."""
@@ -118,6 +133,24 @@ def _break_symmetry(pb, L, seed):
     neg = [l * l + l + m for l in range(1, L + 1) for m in range(-l, 0)]
     Ar[:, neg] *= 1.0 + 1e-3 * rng.standard_normal((Ar.shape[0], len(neg)))
     Ai[:, neg] *= 1.0 + 1e-3 * rng.standard_normal((Ai.shape[0], len(neg)))
+    return dict(pb, A_re=Ar.reshape(np.shape(pb["A_re"])), A_im=Ai.reshape(np.shape(pb["A_im"])))
+
+
+def _break_m0(pb, L, seed):
+    """The problem with only the receptor's m = 0 coefficients moved off the
+    symmetry: A_l0 must be real for even l and imaginary for odd l; a part
+    of 1e-3 of each coefficient's size is added to the other component."""
+    K2 = (L + 1) ** 2
+    rng = np.random.default_rng(seed)
+    Ar = np.array(pb["A_re"], dtype=np.float64).reshape(-1, K2).copy()
+    Ai = np.array(pb["A_im"], dtype=np.float64).reshape(-1, K2).copy()
+    for l in range(L + 1):
+        e = l * l + l
+        mag = np.hypot(Ar[:, e], Ai[:, e])
+        if l % 2 == 0:
+            Ai[:, e] += 1e-3 * mag * rng.standard_normal(Ar.shape[0])
+        else:
+            Ar[:, e] += 1e-3 * mag * rng.standard_normal(Ar.shape[0])
     return dict(pb, A_re=Ar.reshape(np.shape(pb["A_re"])), A_im=Ai.reshape(np.shape(pb["A_im"])))
 
 
@@ -179,6 +212,14 @@ def test_entry_sets_symmetric_and_asymmetric(F):
     assert r_g2["stats"]["cross_entries"] == K2
     _cmp_results(pb2, r_g2, r_o2)
     assert not np.allclose(r_g2["cross"], r_g["cross"], rtol=1e-9, atol=0)
+    # symmetric for m >= 1 but not at m = 0: the full set as well
+    pb3 = _break_m0(pb, L, 5)
+    r_o3 = O.compute_saxs_scores(pb3["index"], pb3["A_re"], pb3["A_im"], pb3["B_re"],
+                                 pb3["B_im"], pb3["params"], pb3["qvals"], zv, L, want_cross=True)
+    r_g3 = F.score(_gpu_problem(F, pb3), pb3["index"], want_cross=True)
+    assert r_g3["stats"]["cross_entries"] == K2
+    _cmp_results(pb3, r_g3, r_o3)
+    assert not np.allclose(r_g3["cross"], r_g["cross"], rtol=1e-9, atol=0)
 
 
 def test_out_of_range_poses_untouched(F):
@@ -187,10 +228,15 @@ def test_out_of_range_poses_untouched(F):
     pb = build_problem(200, 150, L, 0.3, 17, zv, 30, seed=8)
     N = 2 * L + 1
     size5d = (L + 1) ** 2 * N ** 3
-    idx = np.concatenate([pb["index"], [len(zv) * size5d + 5, -3]]).astype(np.int64)
+    # past the last z slot, negative, and z quotients at 2^31 + 5 and 2^32 + 1
+    # (a 32-bit z digit would wrap them to a negative key and to z = 1)
+    bad = [len(zv) * size5d + 5, -3, (2 ** 31 + 5) * size5d + 11, (2 ** 32 + 1) * size5d + 7]
+    idx = np.concatenate([pb["index"], bad]).astype(np.int64)
+    nb = len(bad)
     r = F.score(_gpu_problem(F, pb), idx)
-    assert np.all(np.isnan(r["score"][-2:])) and np.all(np.isnan(r["c1"][-2:]))
-    assert not np.any(np.isnan(r["score"][:-2]))
+    assert np.all(np.isnan(r["score"][-nb:])) and np.all(np.isnan(r["c1"][-nb:]))
+    assert np.all(np.isnan(r["c2"][-nb:]))
+    assert not np.any(np.isnan(r["score"][:-nb]))
     assert r["stats"]["n_scored"] == len(pb["index"])
 
 

@@ -1,0 +1,267 @@
+"""The HIP path on the reference's own known answer and on the workloads'
+own grids, against the oracle (FP64 CPU restatement of the reference).
+
+  * score_conformations (tests/saxs_test.c:177-363): the reference's
+    experimental profile, Euler list and ref_chi, run end to end through
+    the product's readers (sxs_profile_read, fsx_euler_list_read), its
+    parameters (sxs_opt_params_create, whose scoring_helper bins the
+    1,714-point experimental curve onto the 50-point model grid) and the
+    drop-in sxs_compute_saxs_scores on the GPU;
+  * the FP64 and mixed forms at the grid and fit bounds of the single-GPU
+    headline workload (L = 15, 51 q on [0, 0.5], 19 betas, 64-point
+    alpha/gamma grids, c1 in [0.95, 1.05], c2 in [0, 4]);
+  * the batched entry point (fsx_score_batch) against the oracle for each
+    complex, with expansion orders 10, 15 and 20;
+  * the mixed form at the full headline size (210,000 poses) against the
+    FP64 form through size-independent properties.
+
+Bars: FP64 form I(q) and every cross term within 1e-10 relative of the
+oracle, score within 1e-7 relative, c1 / c2 within 1e-6, identical top-100
+ranking; mixed form I(q) within 1e-5 relative, chi within 1e-4 relative,
+identical top-100 ranking up to exact ties (see tests/test_gpu_mixed.py for
+the chi bar).  Against ref_chi, the reference test's own 1e-3 absolute.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problem import build_problem
+from tests.test_gpu_mixed import same_ranking
+
+pytestmark = pytest.mark.gpu
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def F():
+    import libfmftsaxs_amd as F
+    if F.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return F
+
+
+def _I(c):
+    return c[..., 0, :] - c[..., 1, :] + c[..., 2, :] + c[..., 3, :] - c[..., 4, :] + c[..., 5, :]
+
+
+def _fp64_parity(r_g, r_o, tol=1e-10):
+    ok = ~np.isnan(r_o["score"])
+    assert np.array_equal(ok, ~np.isnan(r_g["score"]))
+    cg, co = r_g["cross"][ok], r_o["cross"][ok]
+    relI = np.abs(_I(cg) - _I(co)) / np.abs(_I(co))
+    assert relI.max() <= tol, relI.max()
+    assert (np.abs(cg - co) / np.abs(co).max(axis=2, keepdims=True)).max() <= tol
+    sg, so = r_g["score"][ok], r_o["score"][ok]
+    assert np.all(np.abs(sg - so) <= 1e-7 * so)
+    assert np.all(np.abs(r_g["c1"][ok] - r_o["c1"][ok]) <= 1e-6)
+    assert np.all(np.abs(r_g["c2"][ok] - r_o["c2"][ok]) <= 1e-6)
+    k = min(100, int(ok.sum()))
+    assert np.array_equal(np.argsort(sg, kind="stable")[:k], np.argsort(so, kind="stable")[:k])
+    return relI.max()
+
+
+def _mixed_parity(r_m, r_o):
+    ok = ~np.isnan(r_o["score"])
+    assert np.array_equal(ok, ~np.isnan(r_m["score"]))
+    relI = np.abs(_I(r_m["cross"][ok]) - _I(r_o["cross"][ok])) / np.abs(_I(r_o["cross"][ok]))
+    relc = np.abs(r_m["score"][ok] - r_o["score"][ok]) / r_o["score"][ok]
+    assert relI.max() <= 1e-5, relI.max()
+    assert relc.max() <= 1e-4, relc.max()
+    assert same_ranking(r_m["score"][ok], r_o["score"][ok], min(100, int(ok.sum())))
+    return relI.max(), relc.max()
+
+
+# ---- score_conformations through the product -------------------------------
+def _fixture_mol(name, center):
+    from libfmftsaxs_amd import synthetic as S
+    t = np.load(os.path.join(G, f"atoms_{name}.npz"))
+    m = S.Molecule(t["xyz"], ["C"] * len(t["xyz"]))
+    c = m.center_of_extrema() if center == "coe" else m.centroid()
+    m = m.translated(-c)
+    m.vac, m.dum, m.vdw = t["vac"], t["dum"], t["vdw"]
+    return m
+
+
+def test_ref_chi_through_dropin(F):
+    """saxs_test.c:177-363 on the GPU: receptor amplitudes of the reference
+    (ref_spf), ligand 4g9s_l_moved expanded on the GPU (centroid-centred,
+    Shrake-Rupley SASA for its water shell), the experimental profile read
+    by sxs_profile_read, sxs_opt_params_create with the joint mean radius
+    (correlate.c:125-126), the z = 40 lines of the reference's Euler list
+    snapped by fsx_euler_list_read, and the drop-in
+    sxs_compute_saxs_scores (skip = 1)."""
+    from libfmftsaxs_amd import synthetic as S
+    lib = F.lib()
+    lib.fsx_euler_list_read.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(C.POINTER(C.c_int64)),
+                                        C.POINTER(C.POINTER(C.c_int)),
+                                        C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
+    lib.fsx_free.argtypes = [C.c_void_p]
+    spf = np.load(os.path.join(G, "ref_spf.npz"))
+    L, qnum = int(spf["L"]), int(spf["qnum"])
+    q = F.mkarray(0.0, 0.5, qnum)
+    lig = _fixture_mol("4g9s_l_moved", "centroid")
+    lig.sa = S.shrake_rupley(lig, npts=1500)
+    Br, Bi, rmB = F.atoms2spf(lig.coords, lig.vac, lig.dum, lig.sa, lig.vdw, q, L)
+    nA, nB = 1735, lig.natoms
+    rm = (float(spf["rm"]) * nA + rmB * nB) / (nA + nB)
+    prof = lib.sxs_profile_read(os.path.join(G, "ref_saxs.txt").encode())
+    assert prof and prof.contents.qnum == 1714
+    qc = np.ascontiguousarray(q)
+    params = lib.sxs_opt_params_create(prof, qc.ctypes.data_as(C.POINTER(C.c_double)), qnum, rm)
+    a = np.ctypeslib.as_array(params.contents.a, shape=(6 * qnum,)).copy()
+    ex = np.loadtxt(os.path.join(G, "ref_saxs.txt"))
+    assert np.array_equal(a, O.scoring_helper(ex[:, 0], ex[:, 1], ex[:, 2], q))
+    assert params.contents.peak == ex[0, 1]
+    zv = np.array([40.0])
+    idx = C.POINTER(C.c_int64)()
+    fid = C.POINTER(C.c_int)()
+    order = C.POINTER(C.c_int)()
+    n = C.c_int()
+    assert lib.fsx_euler_list_read(os.path.join(G, "euler_z40.txt").encode(), zv.ctypes.data,
+                                   1, L, L + 1, 2 * L + 1, 2 * L + 1, F.PI_REFERENCE,
+                                   C.byref(idx), C.byref(fid), C.byref(order), C.byref(n)) == 0
+    n = n.value
+    index = np.array([idx[i] for i in range(n)], dtype=np.int64)
+    ft = np.array([fid[i] for i in range(n)])
+    for p in (idx, fid, order):
+        lib.fsx_free(C.cast(p, C.c_void_p))
+    A = F.spf_from_arrays(spf["re"], spf["im"], L, float(spf["rm"]))
+    B = F.spf_from_arrays(Br, Bi, L, rmB)
+    i32 = np.ascontiguousarray(index, dtype=np.int32)
+    sc, c1, c2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    print(f"ref_chi through the drop-in: {n} poses", flush=True)
+    lib.sxs_compute_saxs_scores(sc.ctypes.data, c1.ctypes.data, c2.ctypes.data, i32.ctypes.data,
+                                n, A, B, params, qc.ctypes.data, qnum, zv.ctypes.data, 1, L, 1)
+    ref = np.loadtxt(os.path.join(G, "ref_chi.txt"))
+    assert n == 52 and np.array_equal(ref[:, 0].astype(int), ft)
+    d = np.abs(np.stack([sc - ref[:, 1], c1 - ref[:, 2], c2 - ref[:, 3]]))
+    print(f"max |diff| vs ref_chi: score {d[0].max():.2e}, c1 {d[1].max():.2e}, "
+          f"c2 {d[2].max():.2e}")
+    assert d.max() <= 1e-3 + 1e-9
+    # the oracle on the same inputs
+    P = O.Params(a, rm, ex[0, 1])
+    assert P.mult == params.contents.mult
+    r_o = O.compute_saxs_scores(index, spf["re"], spf["im"], Br, Bi, P, q, zv, L)
+    assert np.all(np.abs(sc - r_o["score"]) <= 1e-7 * r_o["score"])
+    assert np.all(np.abs(c1 - r_o["c1"]) <= 1e-6) and np.all(np.abs(c2 - r_o["c2"]) <= 1e-6)
+    assert np.array_equal(np.argsort(sc, kind="stable"), np.argsort(r_o["score"], kind="stable"))
+    lib.sxs_opt_params_free(params)
+    lib.sxs_profile_free(prof)
+    lib.sxs_spf_full_free(A)
+    lib.sxs_spf_full_free(B)
+
+
+# ---- the headline grid and fit bounds ----------------------------------------
+def _slab_poses(znum, nb, na, ng, slabs, per_slab, seed):
+    """per_slab distinct poses in each of the given (z, b1, b2) slabs (the
+    oracle's cost grows with the number of slabs, not of poses)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for (z, b1, b2) in slabs:
+        base = ((z * nb + b1) * nb + b2) * na * ng * ng
+        out.append(base + rng.choice(na * ng * ng, per_slab, replace=False))
+    idx = np.concatenate(out).astype(np.int64)
+    rng.shuffle(idx)
+    return idx
+
+
+def test_headline_grid_and_bounds(F):
+    """L = 15, 51 q on [0, 0.5], 19 betas (10 degrees), 64-point alpha/gamma
+    grids, z 25..60 A, fits in c1 [0.95, 1.05] and c2 [0, 4]: 6 slabs of 40
+    poses each, the FP64 form at its 1e-10 bar and the mixed form at its
+    bars, both against the oracle with the same bounds."""
+    L, nb, ng = 15, 19, 64
+    zv = np.arange(25.0, 61.0)
+    pb = build_problem(1500, 800, L, 0.5, 51, zv, 1, seed=101, nbeta=nb, na=ng, ng=ng)
+    slabs = [(3, 0, 7), (10, 4, 18), (17, 9, 9), (22, 18, 2), (30, 12, 15), (35, 6, 0)]
+    idx = _slab_poses(zv.size, nb, ng, ng, slabs, 40, 102)
+    c1b, c2b = (0.95, 1.05), (0.0, 4.0)
+    P = O.Params(pb["a"], pb["rm"], pb["peak"], c1=c1b, c2=c2b)
+    O.set_pi(O.PI_TRUNC)
+    r_o = O.compute_saxs_scores(idx, pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"], P,
+                                pb["qvals"], zv, L, nbeta=nb, na=ng, ng=ng, want_cross=True)
+    gp = F.ScoringProblem(L, pb["qvals"], zv, pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                          pb["a"], P.mult, pb["peak"], nbeta=nb, nalpha=ng, ngamma=ng,
+                          c1=c1b, c2=c2b)
+    print("headline grid, FP64 form", flush=True)
+    r_g = F.score(gp, idx, want_cross=True)
+    rel = _fp64_parity(r_g, r_o)
+    # the bounds are active: some fits end on a face of the box
+    ok = ~np.isnan(r_o["c1"])
+    on_face = (np.isin(r_o["c1"][ok], c1b) | np.isin(r_o["c2"][ok], c2b)).mean()
+    print(f"FP64 max rel I {rel:.2e}; fits on a face of the box: {on_face:.2f}")
+    assert on_face > 0
+    print("headline grid, mixed form", flush=True)
+    r_m = F.score(gp, idx, want_cross=True, mixed=True)
+    relI, relc = _mixed_parity(r_m, r_o)
+    print(f"mixed max rel I {relI:.2e}, chi {relc:.2e}")
+
+
+# ---- the batched entry point -------------------------------------------------
+def test_batch_vs_oracle(F):
+    """fsx_score_batch over complexes of expansion orders 10, 15 and 20
+    (different atom counts, q grids, z ranges, grids and pose counts), each
+    complex against the oracle, in both forms."""
+    specs = [dict(n=(400, 250), L=10, Q=21, qmax=0.4, zv=np.arange(22.0, 28.0), npose=80,
+                  nb=11, ng=21, seed=41),
+             dict(n=(600, 300), L=15, Q=25, qmax=0.5, zv=np.arange(25.0, 31.0), npose=60,
+                  nb=19, ng=64, seed=42),
+             dict(n=(700, 400), L=20, Q=17, qmax=0.5, zv=np.arange(30.0, 34.0), npose=40,
+                  nb=21, ng=41, seed=43)]
+    pbs, gps, ros = [], [], []
+    O.set_pi(O.PI_TRUNC)
+    for sp in specs:
+        pb = build_problem(sp["n"][0], sp["n"][1], sp["L"], sp["qmax"], sp["Q"], sp["zv"],
+                           sp["npose"], seed=sp["seed"], nbeta=sp["nb"], na=sp["ng"],
+                           ng=sp["ng"])
+        ros.append(O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"],
+                                         pb["B_im"], pb["params"], pb["qvals"], sp["zv"],
+                                         sp["L"], nbeta=sp["nb"], na=sp["ng"], ng=sp["ng"],
+                                         want_cross=True))
+        gps.append(F.ScoringProblem(sp["L"], pb["qvals"], sp["zv"], pb["A_re"], pb["A_im"],
+                                    pb["B_re"], pb["B_im"], pb["a"], pb["params"].mult,
+                                    pb["peak"], nbeta=sp["nb"], nalpha=sp["ng"],
+                                    ngamma=sp["ng"]))
+        pbs.append(pb)
+    print("batch of 3 (L 10, 15, 20), FP64 form", flush=True)
+    rb = F.score_batch(gps, [pb["index"] for pb in pbs], want_cross=True)
+    for sp, r, ro in zip(specs, rb, ros):
+        rel = _fp64_parity(r, ro)
+        print(f"  L={sp['L']}: FP64 max rel I {rel:.2e}")
+    print("batch of 3 (L 10, 15, 20), mixed form", flush=True)
+    rm = F.score_batch(gps, [pb["index"] for pb in pbs], want_cross=True, mixed=True)
+    for sp, r, ro in zip(specs, rm, ros):
+        relI, relc = _mixed_parity(r, ro)
+        print(f"  L={sp['L']}: mixed max rel I {relI:.2e}, chi {relc:.2e}")
+
+
+# ---- the mixed form at the full headline size ----------------------------------
+def test_mixed_vs_fp64_at_headline_size(F):
+    """The headline workload itself (receptor 3,000 / ligand 1,500 atoms,
+    210,000 poses on the grid above): the oracle cannot run it in seconds, so
+    the mixed form is checked against the FP64 form (itself at the 1e-10 bar
+    above) through size-independent properties: every pose scored and
+    finite, I(q) of every pose within 1e-5 relative, chi within 1e-4, and
+    the identical top-100 ranking up to exact ties."""
+    import bench
+    w = bench.build_workload(**bench.WORKLOADS["standard"])
+    pb, poses = w["problem"], w["poses"]
+    print(f"headline size: {poses.size} poses, FP64 form", flush=True)
+    r64 = F.score(pb, poses, want_cross=True)
+    print("headline size: mixed form", flush=True)
+    r32 = F.score(pb, poses, want_cross=True, mixed=True)
+    for r in (r64, r32):
+        assert r["stats"]["n_scored"] == poses.size
+        assert np.all(np.isfinite(r["score"])) and np.all(np.isfinite(r["cross"]))
+    I64, I32 = _I(r64["cross"]), _I(r32["cross"])
+    relI = np.abs(I32 - I64) / np.abs(I64)
+    relc = np.abs(r32["score"] - r64["score"]) / r64["score"]
+    print(f"mixed vs FP64 at 210,000 poses: max rel I {relI.max():.2e}, chi {relc.max():.2e}")
+    assert relI.max() <= 1e-5 and relc.max() <= 1e-4
+    assert same_ranking(r32["score"], r64["score"], 100)
