@@ -1046,60 +1046,201 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     out[4L * s + 3] = nev;
 }
 
-/* ---- atom expansion (pdb2spf.c:24-152) ------------------------------------ */
-__global__ void k_atom_bessel(double *J, const double *r, const double *qvals, int natoms, int Q,
-                              int K)
+/* ---- atom expansion (pdb2spf.c:24-152) ------------------------------------
+ * A^X_lm(q) = 4 pi i^l sum_a f^X_a j_l(q r_a) conj(Y_lm(theta_a, phi_a)) for
+ * X = V, D, W (f^V, f^D the scalar form factors, f^W = zero_ff(OH2) * SASA).
+ * With real f, conj(Y_{l,-m}) = (-1)^m Y_lm, so the sums are formed for
+ * m >= 0 only and A_{l,-m} = (-1)^m 4 pi i^l conj(S_lm), S_lm the m >= 0 sum.
+ * Stages, per pass over a range of atoms:
+ *   k_atom_geom   r, f^X and conj(Y_lm) (m >= 0) per atom, the reference's
+ *                 angles (cart2sph), norms and Legendre recurrence
+ *                 (borrowed.c:19-65), contraction off
+ *   k_atom_bes    J[l][atom][q] = j_l(q r) by the reference's series, q padded
+ *                 to a multiple of 16 with zeros
+ *   k_atom_mfma   per (chunk of 1024 atoms, l, 16 q): the sums over the chunk's
+ *                 atoms as FP64 MFMA products of the (q x atoms) Bessel tile and
+ *                 the (atoms x (X, m, re/im)) tile f^X conj(Y_lm); four waves
+ *                 take 256 atoms each and are summed in LDS in a fixed order;
+ *                 one partial per chunk
+ * then k_atom_finish sums the chunks in order and applies 4 pi i^l. */
+#define FSX_AT_CHUNK 1024   /* atoms per k_atom_mfma block (4 waves x 256) */
+#define FSX_AT_MAXT 16      /* column tiles: 3 * 2 (L + 1) <= 16 * 16 for L <= 41 */
+typedef double fsx_v4d __attribute__((ext_vector_type(4)));
+
+__global__ void k_atom_geom(const double *xyz, const double *vac, const double *dum,
+                            const double *sa, double h2o_ff, int na, int L, double pi_ref,
+                            double *r_out, double *f_out, double *Y)
 {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)natoms * Q * K) return;
-    int a = (int)(i % natoms);
-    int l = (int)((i / natoms) % K);
-    int q = (int)(i / ((long)natoms * K));
-    J[i] = dev_sbessel(l, qvals[q] * r[a]);
+#pragma clang fp contract(off)
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const int N = L + 1, Kh = N * (N + 1) / 2;
+    /* cart2sph (pdb2spf.c:9-22) */
+    const double x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2];
+    const double rr = sqrt(x * x + y * y + z * z);
+    const double th = acos(z / rr);
+    double fi;
+    if (y > 0.0) fi = acos(x / sqrt(x * x + y * y));
+    else fi = -acos(x / sqrt(x * x + y * y)) + 2.0 * pi_ref;
+    r_out[a] = rr;
+    f_out[a] = vac[a];
+    f_out[na + a] = dum[a];
+    f_out[2 * na + a] = sa ? h2o_ff * sa[a] : 0.0;
+    /* associated Legendre values (borrowed.c:40-65), written as conj(Y_lm)
+     * = yn(l, m) P_l^m (cos m phi, sin(-m phi)) as soon as each is formed;
+     * yn(l, m) = sqrt((2l + 1) / (4 pi) / prod_{k=1..m} (l - k + 1)(l + k))
+     * (borrowed.c:19-38) */
+    double *Ya = Y + (size_t)a * Kh * 2;
+    const double xc = cos(th), yc = sqrt(1. - xc * xc);
+    auto put = [&](int l, int m, double P) {
+        double f = 1.;
+        const double base = (2 * l + 1.) / (4. * pi_ref);
+        for (int k = 1; k <= m; k++) f *= (l - k + 1) * (l + k);
+        const double h = sqrt(base / f) * P;
+        const int e = l * (l + 1) / 2 + m;
+        Ya[2 * e] = h * cos(fi * m);
+        Ya[2 * e + 1] = h * sin(-fi * m);
+    };
+    /* m = 0 column */
+    double p2 = 1, p1 = xc;
+    put(0, 0, 1.0);
+    if (N > 1) put(1, 0, xc);
+    for (int l = 2; l < N; ++l) {
+        const double p = (2. * l - 1.) / l * xc * p1 - (l - 1.) / l * p2;
+        put(l, 0, p);
+        p2 = p1;
+        p1 = p;
+    }
+    /* m >= 1: the diagonal from the previous one, then upwards in l */
+    double pmm = 1.0;
+    for (int m = 1; m < N - 1; ++m) {
+        pmm = -1. * (2. * m - 1.) * yc * pmm;
+        put(m, m, pmm);
+        double q2 = pmm, q1 = (2. * m + 1.) * xc * pmm;
+        put(m + 1, m, q1);
+        for (int l = m + 2; l < N; ++l) {
+            const double p = (2. * l - 1.) / (l - m) * xc * q1 - (l + m - 1.) / (l - m) * q2;
+            put(l, m, p);
+            q2 = q1;
+            q1 = p;
+        }
+    }
+    if (N > 1) put(N - 1, N - 1, -1. * (2. * (N - 1.) - 1.) * yc * pmm);
 }
 
-/* one block per (q, lm): sum over atoms of f^X_a j_l(q r_a) conj(Y_lm), then
- * the 4 pi i^l factor (pdb2spf.c:118-147) */
-__global__ void __launch_bounds__(256)
-k_atom_sum(double *re, double *im, const double *J, const double *Y, const double *f,
-           int natoms, int Q, int L, double four_pi)
+/* J[l][a][q] = j_l(q r_a) for q < Q, 0 for Q <= q < Qp */
+__global__ void k_atom_bes(double *J, const double *r, const double *qvals, int na, int Q, int Qp,
+                           int K)
 {
-    __shared__ double red[6][4];
-    const int K = L + 1, K2 = K * K;
-    int lm = blockIdx.x % K2, q = blockIdx.x / K2;
-    int l = lm_l(lm);
-    const double *Jq = J + ((long)q * K + l) * natoms;
-    const double *yr = Y + (long)lm * natoms, *yi = Y + (long)(K2 + lm) * natoms;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
-        double j = Jq[a];
-        double vr = j * yr[a], vi = j * yi[a];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)K * na * Qp) return;
+    const int q = (int)(i % Qp);
+    const long la = i / Qp;
+    const int a = (int)(la % na), l = (int)(la / na);
+    J[i] = q < Q ? dev_sbessel(l, qvals[q] * r[a]) : 0.0;
+}
+
+/* One block of four waves per (chunk, l, 16-q tile).  Rows are q, columns
+ * c = X * 2(l+1) + 2m + part (part 0: real, 1: imaginary) of f^X conj(Y_lm).
+ * FP64 MFMA 16x16x4 operand layout on gfx950: lane i feeds A[i % 16][i / 16]
+ * and B[i / 16][i % 16]; result register j of lane i is D[i / 16 + 4j][i % 16].
+ * Writes the chunk's partial sums P[chunk][X][Qp][Kh][2]. */
+__global__ void __launch_bounds__(256)
+k_atom_mfma(const double *J, const double *f, const double *Y, double *P, int na, int Qp, int L,
+            int chunk0)
+{
+    __shared__ fsx_v4d red[3][64];
+    const int K = L + 1, Kh = K * (K + 1) / 2, nqt = Qp / 16;
+    int bid = blockIdx.x;
+    const int qt = bid % nqt;
+    bid /= nqt;
+    const int l = bid % K;
+    const int ch = bid / K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ncol = 3 * 2 * (l + 1), nct = (ncol + 15) / 16;
+    /* this lane's column of each tile: X, entry offset and part */
+    int cx[FSX_AT_MAXT], coff[FSX_AT_MAXT];
 #pragma unroll
-        for (int X = 0; X < 3; X++) {
-            double fx = f[(long)X * natoms + a];
-            acc[2 * X] += fx * vr;
-            acc[2 * X + 1] += fx * vi;
+    for (int t = 0; t < FSX_AT_MAXT; t++) {
+        const int c = t * 16 + (lane & 15);
+        const int X = c / (2 * (l + 1)), rem = c - X * 2 * (l + 1);
+        cx[t] = c < ncol ? X : -1;
+        coff[t] = c < ncol ? l * (l + 1) / 2 * 2 + rem : 0;   /* (entry, part) as a double index */
+    }
+    fsx_v4d acc[FSX_AT_MAXT];
+#pragma unroll
+    for (int t = 0; t < FSX_AT_MAXT; t++) acc[t] = (fsx_v4d){0.0, 0.0, 0.0, 0.0};
+    const int abeg = ch * FSX_AT_CHUNK + w * (FSX_AT_CHUNK / 4);
+    const int aend = min(abeg + FSX_AT_CHUNK / 4, na);
+    const double *Jl = J + (size_t)l * na * Qp + qt * 16 + (lane & 15);
+    for (int a0 = abeg; a0 < aend; a0 += 4) {
+        const int a = a0 + (lane >> 4);
+        const bool ok = a < aend;
+        const double av = ok ? Jl[(size_t)a * Qp] : 0.0;
+        const double *Ya = Y + (size_t)(ok ? a : 0) * Kh * 2;
+#pragma unroll
+        for (int t = 0; t < FSX_AT_MAXT; t++) {
+            if (t < nct) {
+                const double bv = ok && cx[t] >= 0 ? f[(size_t)cx[t] * na + a] * Ya[coff[t]] : 0.0;
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+            }
         }
     }
-    int wave = threadIdx.x / FSX_WAVE, lane = threadIdx.x % FSX_WAVE;
+    /* wave 0 + wave 1 + wave 2 + wave 3, in that order, per tile */
+    const int q0 = qt * 16;
+    double *Pc = P + (size_t)(chunk0 + ch) * 3 * Qp * Kh * 2;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        double v = wave_sum(acc[k]);
-        if (lane == 0) red[k][wave] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        int X = threadIdx.x;
-        double vr = 0.0, vi = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / FSX_WAVE); w++) {
-            vr += red[2 * X][w];
-            vi += red[2 * X + 1][w];
+    for (int t = 0; t < FSX_AT_MAXT; t++) {
+        if (t < nct) {
+            if (w > 0) red[w - 1][lane] = acc[t];
+            __syncthreads();
+            if (w == 0) {
+                fsx_v4d v = acc[t];
+                v += red[0][lane];
+                v += red[1][lane];
+                v += red[2][lane];
+                if (cx[t] >= 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int q = q0 + (lane >> 4) + 4 * j;
+                        Pc[((size_t)cx[t] * Qp + q) * Kh * 2 + coff[t]] = v[j];
+                    }
+                }
+            }
+            __syncthreads();
         }
-        const double cr[4] = {four_pi, 0.0, -four_pi, 0.0}, ci[4] = {0.0, four_pi, 0.0, -four_pi};
-        double c_r = cr[l % 4], c_i = ci[l % 4];
-        long o = ((long)X * Q + q) * K2 + lm;
-        re[o] = c_r * vr - c_i * vi;
-        im[o] = c_r * vi + c_i * vr;
+    }
+}
+
+/* A^X_lm = 4 pi i^l S_lm and A^X_{l,-m} = (-1)^m 4 pi i^l conj(S_lm), S the
+ * sum of the nch chunk partials in chunk order; one thread per (X, q, l, m >= 0) */
+__global__ void k_atom_finish(const double *P, int nch, double *re, double *im, int Q, int Qp,
+                              int L, double four_pi)
+{
+#pragma clang fp contract(off)
+    const int K = L + 1, Kh = K * (K + 1) / 2, K2 = K * K;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3L * Q * Kh) return;
+    const int e = (int)(i % Kh);
+    const int xq = (int)(i / Kh), q = xq % Q, X = xq / Q;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= e) l++;
+    const int m = e - l * (l + 1) / 2;
+    double sr = 0.0, si = 0.0;
+    const size_t off = (((size_t)X * Qp + q) * Kh + e) * 2, stride = (size_t)3 * Qp * Kh * 2;
+    for (int c = 0; c < nch; c++) {
+        sr += P[c * stride + off];
+        si += P[c * stride + off + 1];
+    }
+    const double cr[4] = {four_pi, 0.0, -four_pi, 0.0}, ci[4] = {0.0, four_pi, 0.0, -four_pi};
+    const double c_r = cr[l % 4], c_i = ci[l % 4];
+    const size_t o = ((size_t)X * Q + q) * K2;
+    re[o + l * (l + 1) + m] = c_r * sr - c_i * si;
+    im[o + l * (l + 1) + m] = c_r * si + c_i * sr;
+    if (m > 0) {
+        const double sg = (m & 1) ? -1.0 : 1.0, mr = sg * sr, mi = -sg * si;
+        re[o + l * (l + 1) - m] = c_r * mr - c_i * mi;
+        im[o + l * (l + 1) - m] = c_r * mi + c_i * mr;
     }
 }
 
@@ -1112,8 +1253,8 @@ static hipStream_t g_stream;
 enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
-    B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AR, B_AF,
-    B_AY, B_AJ, B_AQ, B_ARE, B_AIM, B_RTMP, B_FITSC, B_NBUF
+    B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AXYZ,
+    B_AFF, B_AWS, B_AQ, B_ARE, B_AIM, B_RTMP, B_FITSC, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -1798,39 +1939,89 @@ extern "C" int fsx_hip_sbessel_batch(const double *x, int n, int pmax, double *o
     return rc;
 }
 
-extern "C" int fsx_hip_atoms2spf(int natoms, const double *r, const double *f, const double *Y,
+/* Atom-expansion workspace: the per-chunk partial sums of all natoms atoms,
+ * then the tables of one pass of atoms (Bessel table, f, r, conj(Y)). */
+static inline int at_qp(int Q) { return (Q + 15) / 16 * 16; }
+static inline size_t at_part_bytes(int natoms, int Q, int L)
+{
+    const size_t Kh = (size_t)(L + 1) * (L + 2) / 2, nch = ((size_t)natoms + FSX_AT_CHUNK - 1) / FSX_AT_CHUNK;
+    return (nch * 3 * at_qp(Q) * Kh * 2 * sizeof(double) + 255) & ~(size_t)255;
+}
+static inline size_t at_atom_bytes(int Q, int L)
+{
+    const size_t K = L + 1, Kh = K * (K + 1) / 2;
+    return sizeof(double) * (K * at_qp(Q) + 4 + 2 * Kh);
+}
+
+/* The expansion of natoms atoms (device inputs) into re/im [3][Q][(L+1)^2]
+ * (device), on stream s, in ws (ws_bytes): passes of as many whole chunks of
+ * atoms as the workspace holds. */
+static int atoms_run(hipStream_t s, char *ws, size_t ws_bytes, int natoms, const double *xyz,
+                     const double *vac, const double *dum, const double *sa, double h2o_ff,
+                     const double *qv, int Q, int L, double *re, double *im)
+{
+    const int K = L + 1, Kh = K * (K + 1) / 2, Qp = at_qp(Q);
+    const int nch = (natoms + FSX_AT_CHUNK - 1) / FSX_AT_CHUNK;
+    const size_t pb = at_part_bytes(natoms, Q, L), per = at_atom_bytes(Q, L);
+    if (ws_bytes < pb + per * FSX_AT_CHUNK && ws_bytes < pb + per * (size_t)natoms) return FSX_ERR_ARG;
+    long npass = (long)((ws_bytes - pb) / per) / FSX_AT_CHUNK * FSX_AT_CHUNK;
+    if (npass < 1) npass = natoms;   /* a single pass of fewer than one chunk of atoms */
+    if (npass > natoms) npass = natoms;
+    double *P = reinterpret_cast<double *>(ws);
+    double *J = reinterpret_cast<double *>(ws + pb);
+    double *f = J + (size_t)K * npass * Qp, *r = f + 3 * (size_t)npass, *Y = r + npass;
+    for (long a0 = 0; a0 < natoms; a0 += npass) {
+        const int na = (int)(natoms - a0 < npass ? natoms - a0 : npass);
+        hipLaunchKernelGGL(k_atom_geom, dim3(nblk(na, 256)), dim3(256), 0, s, xyz + 3 * a0,
+                           vac + a0, dum + a0, sa ? sa + a0 : (const double *)NULL, h2o_ff, na, L,
+                           FSX_PI_REFERENCE, r, f, Y);
+        hipLaunchKernelGGL(k_atom_bes, dim3(nblk((long)K * na * Qp, 256)), dim3(256), 0, s, J,
+                           (const double *)r, qv, na, Q, Qp, K);
+        const int nchp = (na + FSX_AT_CHUNK - 1) / FSX_AT_CHUNK;
+        hipLaunchKernelGGL(k_atom_mfma, dim3((unsigned)(nchp * K * (Qp / 16))), dim3(256), 0, s,
+                           (const double *)J, (const double *)f, (const double *)Y, P, na, Qp, L,
+                           (int)(a0 / FSX_AT_CHUNK));
+    }
+    hipLaunchKernelGGL(k_atom_finish, dim3(nblk(3L * Q * Kh, 256)), dim3(256), 0, s,
+                       (const double *)P, nch, re, im, Q, Qp, L, 4.0 * FSX_PI_REFERENCE);
+    return hipGetLastError() == hipSuccess ? FSX_OK : FSX_ERR_HIP;
+}
+
+extern "C" int fsx_hip_atoms2spf(int natoms, const double *xyz, const double *vac,
+                                 const double *dum, const double *sa, double h2o_ff,
                                  const double *qvals, int Q, int L, double *re, double *im)
 {
     pthread_mutex_lock(&g_mtx);
     const int K = L + 1, K2 = K * K;
+    const size_t wsb = at_part_bytes(natoms, Q, L) + at_atom_bytes(Q, L) * (size_t)natoms;
     int rc = init_once();
-    if (rc == FSX_OK) rc = ensure(B_AR, sizeof(double) * natoms);
-    if (rc == FSX_OK) rc = ensure(B_AF, sizeof(double) * 3 * (size_t)natoms);
-    if (rc == FSX_OK) rc = ensure(B_AY, sizeof(double) * 2 * (size_t)K2 * natoms);
-    if (rc == FSX_OK) rc = ensure(B_AJ, sizeof(double) * (size_t)Q * K * natoms);
+    if (rc == FSX_OK) rc = ensure(B_AXYZ, sizeof(double) * 3 * (size_t)natoms);
+    if (rc == FSX_OK) rc = ensure(B_AFF, sizeof(double) * 3 * (size_t)natoms);
+    if (rc == FSX_OK) rc = ensure(B_AWS, wsb);
     if (rc == FSX_OK) rc = ensure(B_AQ, sizeof(double) * Q);
     if (rc == FSX_OK) rc = ensure(B_ARE, sizeof(double) * 3 * (size_t)Q * K2);
     if (rc == FSX_OK) rc = ensure(B_AIM, sizeof(double) * 3 * (size_t)Q * K2);
     if (rc == FSX_OK) {
         hipStream_t s = g_stream;
-        hipError_t e = hipMemcpyAsync(g_buf[B_AR], r, sizeof(double) * natoms, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AF], f, sizeof(double) * 3 * (size_t)natoms, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AY], Y, sizeof(double) * 2 * (size_t)K2 * natoms, hipMemcpyHostToDevice, s);
+        double *dff = (double *)g_buf[B_AFF];
+        const size_t nb = sizeof(double) * (size_t)natoms;
+        hipError_t e = hipMemcpyAsync(g_buf[B_AXYZ], xyz, 3 * nb, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dff, vac, nb, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dff + natoms, dum, nb, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && sa) e = hipMemcpyAsync(dff + 2 * (size_t)natoms, sa, nb, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AQ], qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
-            long nj = (long)Q * K * natoms;
-            hipLaunchKernelGGL(k_atom_bessel, dim3(nblk(nj, 256)), dim3(256), 0, s,
-                               (double *)g_buf[B_AJ], (const double *)g_buf[B_AR],
-                               (const double *)g_buf[B_AQ], natoms, Q, K);
-            hipLaunchKernelGGL(k_atom_sum, dim3((unsigned)(Q * K2)), dim3(256), 0, s,
-                               (double *)g_buf[B_ARE], (double *)g_buf[B_AIM],
-                               (const double *)g_buf[B_AJ], (const double *)g_buf[B_AY],
-                               (const double *)g_buf[B_AF], natoms, Q, L,
-                               4.0 * FSX_PI_REFERENCE);
-            e = hipGetLastError();
+            rc = atoms_run(s, (char *)g_buf[B_AWS], g_cap[B_AWS], natoms,
+                           (const double *)g_buf[B_AXYZ], dff, dff + natoms,
+                           sa ? dff + 2 * (size_t)natoms : NULL, h2o_ff,
+                           (const double *)g_buf[B_AQ], Q, L, (double *)g_buf[B_ARE],
+                           (double *)g_buf[B_AIM]);
+            if (rc == FSX_ERR_HIP) e = hipErrorUnknown;
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(re, g_buf[B_ARE], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(im, g_buf[B_AIM], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && rc == FSX_OK)
+            e = hipMemcpyAsync(re, g_buf[B_ARE], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && rc == FSX_OK)
+            e = hipMemcpyAsync(im, g_buf[B_AIM], sizeof(double) * 3 * (size_t)Q * K2, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             fprintf(stderr, "[fsx] atoms2spf: %s\n", hipGetErrorString(e));

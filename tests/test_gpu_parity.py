@@ -284,15 +284,17 @@ def test_dropin_sxs_compute_saxs_scores(F):
     lib.sxs_spf_full_free(B)
 
 
-@pytest.mark.parametrize("L", [6, 15])
-def test_atoms2spf_vs_oracle(F, L):
-    """GPU atom expansion (pdb2spf.c:24-152) against the oracle restatement;
-    only the order of the sum over atoms differs."""
+@pytest.mark.parametrize("L,natoms,Q", [(6, 700, 26), (15, 700, 26), (25, 2500, 41)])
+def test_atoms2spf_vs_oracle(F, L, natoms, Q):
+    """GPU atom expansion (pdb2spf.c:24-152: angles, Legendre values and
+    Bessel table on the device, sums over atoms on FP64 MFMA in chunks of
+    1024 atoms) against the oracle restatement; only the order of the sums
+    over atoms differs.  2500 atoms: three chunks, the last one partial."""
     from libfmftsaxs_amd import synthetic as S
-    rec = S.random_chain(700, 17)
+    rec = S.random_chain(natoms, 17)
     rec = rec.translated(-rec.center_of_extrema())
     rec.sa = S.shrake_rupley(rec)
-    q = O.mkarray(0.0, 0.5, 26)
+    q = O.mkarray(0.0, 0.5, Q)
     rg, ig, rmg = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
     ro, io, rmo = O.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L)
     assert rmg == rmo
