@@ -37,6 +37,8 @@ extern "C" {
 #define FSX_ERR_HIP (-2)     /* a HIP runtime call failed */
 #define FSX_ERR_NOMEM (-3)   /* host or device allocation failed */
 #define FSX_ERR_NODEV (-4)   /* no HIP device visible */
+#define FSX_ERR_SCRATCH (-5) /* problem too large for the fixed working memory of
+                                fsx_atoms2spf_noalloc */
 
 /* flags for fsx_score */
 #define FSX_DEVICE_POSES 1u    /* poses points to device memory */
@@ -134,6 +136,27 @@ int fsx_fit_batch(const double *ct, int n, const double *qvals, int qnum, const 
 int fsx_atoms2spf(int natoms, const double *coords, const double *vac, const double *dum,
                   const double *sa, double h2o_ff, const double *vdw, const double *qvals,
                   int qnum, int L, double *re, double *im, double *rm);
+
+/* Allocation-free form of fsx_atoms2spf.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid allocation).  Every pointer
+ * is device memory: coords [natoms][3] (already centred), vac, dum, sa
+ * ([natoms]; sa may be NULL: no water shell), qvals [qnum]; re/im
+ * [3][qnum][(L+1)^2] receive the amplitudes (the mean radius is not
+ * formed).  Working memory is a fixed device array of 64 MiB per GPU, held
+ * until the process ends; a problem that does not fit it (it needs
+ * 24 qnum16 (L+1)(L+2) bytes per started 1024 atoms for the partial sums,
+ * plus 8 (L+1) qnum16 + 8 (L+1)(L+2) + 32 bytes per atom for a pass of
+ * min(natoms, 1024) atoms; qnum16 is qnum rounded up to a multiple of 16)
+ * returns FSX_ERR_SCRATCH.
+ * The call runs on the library's stream and returns when the results are
+ * written; device inputs must be ready.  Calls share the one array per GPU:
+ * the library's lock orders calls made from several threads of one process,
+ * and any other overlap of calls on one GPU would race.  The library's
+ * first call (of any entry point) creates its stream. */
+int fsx_atoms2spf_noalloc(int natoms, const double *coords, const double *vac,
+                          const double *dum, const double *sa, double h2o_ff,
+                          const double *qvals, int qnum, int L, double *re, double *im);
 
 /* Brute-force per-pose scoring, the loop of tools/score_ft_naive.c:156-212
  * as a library call.  xyz [nrec + nlig][3]: receptor atoms first, then ligand

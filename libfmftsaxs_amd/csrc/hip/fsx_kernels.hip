@@ -1250,6 +1250,16 @@ __global__ void k_atom_finish(const double *P, int nch, double *re, double *im, 
 static pthread_mutex_t g_mtx = PTHREAD_MUTEX_INITIALIZER;
 static int g_init = 0;
 static hipStream_t g_stream;
+
+/* Working memory of the allocation-free atom expansion
+ * (fsx_atoms2spf_noalloc): a fixed device array, one copy per GPU, held until
+ * the process ends.  In host code the name scratch is only a host-side
+ * stand-in for the symbol: kernels and copies get g_scratch, its device
+ * address on the library's device, taken once by hipGetSymbolAddress in
+ * init_once. */
+#define FSX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) unsigned char scratch[FSX_NOALLOC_SCRATCH_BYTES];
+static void *g_scratch = NULL;
 enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
@@ -1314,6 +1324,7 @@ static int init_once(void)
         g_nxcd < 1)
         g_nxcd = 1;
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    HIPCHK(hipGetSymbolAddress(&g_scratch, HIP_SYMBOL(scratch)));
     g_init = 1;
     return FSX_OK;
 }
@@ -2031,3 +2042,39 @@ extern "C" int fsx_hip_atoms2spf(int natoms, const double *xyz, const double *va
     pthread_mutex_unlock(&g_mtx);
     return rc;
 }
+
+/* The atom expansion on caller-owned device arrays with no allocation: its
+ * tables and chunk partials live in the fixed scratch array.  Nothing there
+ * needs clearing first: every pass writes its tables whole, and every chunk
+ * partial is written (all of (X, q, l, m, part)) before the final sum reads
+ * it.  The library's lock orders calls from several threads of one process. */
+extern "C" int fsx_atoms2spf_noalloc(int natoms, const double *coords, const double *vac,
+                                     const double *dum, const double *sa, double h2o_ff,
+                                     const double *qvals, int qnum, int L, double *re, double *im)
+{
+    if (natoms < 1 || !coords || !vac || !dum || !qvals || qnum < 1 || L < 0 || L > 40 || !re ||
+        !im)
+        return FSX_ERR_ARG;
+    const size_t need = at_part_bytes(natoms, qnum, L) +
+                        at_atom_bytes(qnum, L) * (size_t)(natoms < FSX_AT_CHUNK ? natoms : FSX_AT_CHUNK);
+    if (need > FSX_NOALLOC_SCRATCH_BYTES) {
+        fprintf(stderr, "[fsx] atoms2spf_noalloc: %d atoms at L = %d with %d q need %zu bytes of "
+                "working memory, more than the %zu-byte scratch array\n", natoms, L, qnum, need,
+                FSX_NOALLOC_SCRATCH_BYTES);
+        return FSX_ERR_SCRATCH;
+    }
+    pthread_mutex_lock(&g_mtx);
+    int rc = init_once();
+    if (rc == FSX_OK) {
+        rc = atoms_run(g_stream, (char *)g_scratch, FSX_NOALLOC_SCRATCH_BYTES, natoms, coords, vac,
+                       dum, sa, sa ? h2o_ff : 0.0, qvals, qnum, L, re, im);
+        const hipError_t e = hipStreamSynchronize(g_stream);
+        if (rc == FSX_OK && e != hipSuccess) {
+            fprintf(stderr, "[fsx] atoms2spf_noalloc: %s\n", hipGetErrorString(e));
+            rc = FSX_ERR_HIP;
+        }
+    }
+    pthread_mutex_unlock(&g_mtx);
+    return rc;
+}
+

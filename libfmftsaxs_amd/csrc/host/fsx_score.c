@@ -35,6 +35,7 @@ const char *fsx_error_string(int code)
     case FSX_ERR_HIP: return "HIP runtime error";
     case FSX_ERR_NOMEM: return "out of memory";
     case FSX_ERR_NODEV: return "no HIP device";
+    case FSX_ERR_SCRATCH: return "problem too large for the fixed working memory";
     default: return "unknown error";
     }
 }
