@@ -41,6 +41,8 @@ typedef struct fsx_hip_args {
 } fsx_hip_args;
 
 int fsx_hip_run(const fsx_hip_args *args, fsx_stats *st);
+/* nprob complexes in one call (stats: nprob records, stage times not set) */
+int fsx_hip_run_batch(int nprob, const fsx_hip_args *args, fsx_stats *st);
 int fsx_hip_fit_batch(const double *ct, int n, const double *qvals, int qnum, const double *a,
                       double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
                       double *out);

@@ -1336,9 +1336,11 @@ extern "C" int fsx_hip_device_count(void)
     return nd;
 }
 
+static void batch_release(void);
 extern "C" void fsx_hip_release(void)
 {
     pthread_mutex_lock(&g_mtx);
+    batch_release();
     for (int i = 0; i < B_NBUF; i++) {
         if (g_buf[i]) (void)hipFree(g_buf[i]);
         g_buf[i] = NULL;
@@ -1356,23 +1358,18 @@ static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t);
         if (r_ != FSX_OK) return r_;   \
     } while (0)
 
-/* launches of the element-type dependent stages (FP64 or FP32 tables) */
-template <typename E>
-static void launch_tmat(hipStream_t s, void *T, long nt, int zn, int Q, int L)
-{
-    hipLaunchKernelGGL(k_tmat<E>, dim3(nblk(nt, 256)), dim3(256), 0, s, (decltype(E::re) *)T,
-                       (const double *)g_buf[B_BES], (const double *)g_buf[B_DS], zn, Q, L);
-}
-
+/* launches of the element-type dependent stages (FP64 or FP32 tables);
+ * g: the buffers of the plan being launched (indices P_*) */
+enum { G_DW = 7, G_ENT = 23, G_COFF = 25 };
 template <typename E>
 static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct, long nrot,
                           size_t lds, int nb, int ng, int Q, int L, int gc, int C,
                           const int *csz, const int *cst, const long *coff, int mlo, int npk0,
-                          int lrot)
+                          int lrot, void *const *g)
 {
     hipLaunchKernelGGL(k_rotate<E>, dim3((unsigned)nrot), dim3(256), lds, s, (E *)tmp, srct,
-                       (const double *)g_buf[B_DW], (const int *)g_buf[B_ENT],
-                       (const long *)g_buf[B_COFF], nb, ng, Q, L, gc, mlo);
+                       (const double *)g[G_DW], (const int *)g[G_ENT],
+                       (const long *)g[G_COFF], nb, ng, Q, L, gc, mlo);
     /* 64-row tiles of FP32 tables, 32-row tiles of FP64 tables (33 KB of LDS) */
     const int TR = sizeof(E) == 8 ? 64 : 32;
     const int nrows = nb * ng;
@@ -1383,18 +1380,22 @@ static void launch_rotate(hipStream_t s, void *out, void *tmp, const cplx *srct,
         if (sizeof(E) == 8)
             hipLaunchKernelGGL((k_tab_transpose<E, 64>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0, (const int *)g_buf[B_ENT] + cst[c], lrot);
+                               c == 0 ? npk0 : 0, (const int *)g[G_ENT] + cst[c], lrot);
         else
             hipLaunchKernelGGL((k_tab_transpose<E, 32>), dim3((unsigned)nblkT), dim3(256), 0, s,
                                (E *)out + coff[c], (const E *)tmp + coff[c], csz[c], nrows, nte, ntr,
-                               c == 0 ? npk0 : 0, (const int *)g_buf[B_ENT] + cst[c], lrot);
+                               c == 0 ? npk0 : 0, (const int *)g[G_ENT] + cst[c], lrot);
     }
 }
 
+/* plan buffer indices the cross-term launch reads (kept equal to the P_*
+ * enumeration below) */
+enum { G_T = 1, G_RT = 2, G_BT = 3, G_COUNTS = 17, G_OFFS = 18, G_RECS = 20, G_ACC = 21,
+       G_TW2 = 24 };
 template <typename E, int G>
 static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                            const long *coff, int Tn, long grid, int nrows, int Q, int L,
-                           int nb, int npk0, int ng, int zn)
+                           int nb, int npk0, int ng, int zn, void *const *g)
 {
     const int tst = (long)Tn * (long)(sizeof(E) / 2) <= 40 * 1024;
     const long ntiles = (long)Q * nb * ((ng + G - 1) / G);
@@ -1426,41 +1427,41 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
                      (sizeof(decltype(E::re)) * 6 + sizeof(int2)) * FSX_CROSS_PB;
         /* persistent-style grid: the blocks that fit the chip at once
          * (FSX_CROSS_BLOCKS_PER_CU overrides the per-CU count) */
-        long g = grid;
-        if (g <= 0) {
+        long gsz = grid;
+        if (gsz <= 0) {
             int per = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, FSX_CROSS_BLOCK, lds)
                     != hipSuccess || per < 1)
                 per = 2;
-            g = (long)g_ncu * per;
+            gsz = (long)g_ncu * per;
         }
-        if (g > ntiles) g = ntiles;
+        if (gsz > ntiles) gsz = ntiles;
         /* XCD-contiguous tile ranges when the grid divides evenly
          * (FSX_CROSS_XCD=0 turns them off) */
         int nx = env_int("FSX_CROSS_XCD", 1, 0, 1) ? g_nxcd : 1;
-        if (nx > 1 && (g % nx != 0 || ntiles < 4L * g)) nx = 1;
-        hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(FSX_CROSS_BLOCK), lds, s,
-                           (double *)g_buf[B_ACC], (const E *)g_buf[B_RT] + coff[c],
-                           (const E *)g_buf[B_BT] + coff[c], (const decltype(E::re) *)g_buf[B_T],
-                           (const E *)g_buf[B_TW2], (const int *)g_buf[B_ENT], cst[c], csz[c],
-                           (const int *)g_buf[B_COUNTS], (const int *)g_buf[B_OFFS],
-                           (const int4 *)g_buf[B_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
+        if (nx > 1 && (gsz % nx != 0 || ntiles < 4L * gsz)) nx = 1;
+        hipLaunchKernelGGL(kern, dim3((unsigned)gsz), dim3(FSX_CROSS_BLOCK), lds, s,
+                           (double *)g[G_ACC], (const E *)g[G_RT] + coff[c],
+                           (const E *)g[G_BT] + coff[c], (const decltype(E::re) *)g[G_T],
+                           (const E *)g[G_TW2], (const int *)g[G_ENT], cst[c], csz[c],
+                           (const int *)g[G_COUNTS], (const int *)g[G_OFFS],
+                           (const int4 *)g[G_RECS], ntiles, nrows, Q, L, nb, nx, ng, zn,
                            c == 0, npk);
     }
 }
 
-/* G = gamma2 rows per tile: FSX_CROSS_ROWS, else 4 for FP32 tables (the
- * translation block and the pose loop shared by more rows) and 2 for FP64
- * tables (LDS-bound) */
+/* G = gamma2 rows per tile: tile_rows when set (the batch tuner's choice),
+ * else FSX_CROSS_ROWS, else 4 for FP32 tables (the translation block and the
+ * pose loop shared by more rows) and 2 for FP64 tables (LDS-bound) */
 template <typename E>
 static void launch_cross(hipStream_t s, int C, const int *csz, const int *cst, const long *coff,
                          int Tn, long grid, int nrows, int Q, int L, int nb, int npk0, int ng,
-                         int zn)
+                         int zn, void *const *g, int tile_rows)
 {
-    const int G = env_int("FSX_CROSS_ROWS", sizeof(E) == 8 ? 4 : 2, 1, 4);
-    if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
-    else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
-    else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn);
+    const int G = tile_rows > 0 ? tile_rows : env_int("FSX_CROSS_ROWS", sizeof(E) == 8 ? 4 : 2, 1, 4);
+    if (G >= 4) launch_cross_g<E, 4>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn, g);
+    else if (G >= 2) launch_cross_g<E, 2>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn, g);
+    else launch_cross_g<E, 1>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk0, ng, zn, g);
 }
 
 /* Entry tables of the m-major coefficient order used by the rotated tables
@@ -1534,53 +1535,111 @@ static int spf_conj_symmetric(const double *re, const double *im, int Q, int L)
     return 1;
 }
 
-static int run_locked(const fsx_hip_args *A, fsx_stats *st)
+/* ---- one complex's pipeline: plan, buffers, uploads, launches ---------------
+ * plan_init derives sizes and the host tables of a call; plan_bind gets its
+ * device buffers (the process-wide slots for single calls, per-complex sets
+ * for batches); plan_upload copies the inputs and tables; plan_enqueue
+ * launches every stage on one stream without waiting; plan_finish fills the
+ * statistics once the stream has drained. */
+enum {
+    P_BES, P_T, P_RT, P_BT, P_RTMP, P_A, P_B, P_DW, P_DS, P_TWA, P_TWG, P_Q, P_Z, P_AM, P_CINT,
+    P_KEYS, P_INFO, P_COUNTS, P_OFFS, P_FILL, P_RECS, P_ACC, P_FITSC, P_ENT, P_TW2, P_COFF,
+    P_NEXT, P_TALLY, P_SRCT, P_POSES, P_SCORE, P_C1, P_C2, P_CROSSOUT, P_NBUF
+};
+/* the process-wide slot of each plan buffer (single calls) */
+static const int k_plan_slot[P_NBUF] = {
+    B_BES, B_T, B_RT, B_BT, B_RTMP, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
+    B_KEYS, B_INFO, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_ACC, B_FITSC, B_ENT, B_TW2, B_COFF,
+    B_NEXT, B_TALLY, B_SRCT, B_POSES, B_SCORE, B_C1, B_C2, B_CROSSOUT};
+
+typedef void (*fit_kernel_t)(const double *, const int4 *, const int *, int, const double *,
+                             const double *, double, double, double, double, double, double *,
+                             double *, double *, int *, unsigned long long *);
+
+struct fsx_plan {
+    fsx_hip_args A;
+    int L, K, K2, N, Q, n, nb, na, ng, zn, nrows, mixed, half, mlo, npk, C, Tn, ne, dev_out;
+    long nkeys;
+    size_t dsz, spf, es, nent;
+    int *ent;     /* host tables (malloc) */
+    long *coff;
+    double *tw2;
+    size_t sz[P_NBUF];   /* bytes of each device buffer (0: not used) */
+    void *d[P_NBUF];     /* device buffers */
+    const long long *dposes;
+    double *dscore, *dc1, *dc2, *dcross;
+    int aq_lds;
+    size_t lds_fit;
+    long lanes;
+    fit_kernel_t kfit;
+    int tile_rows;       /* gamma2 rows per cross-term tile (0: the form's default) */
+};
+
+static void plan_free_host(fsx_plan *p)
 {
-    RC(init_once());
+    free(p->ent);
+    free(p->coff);
+    free(p->tw2);
+    p->ent = NULL;
+    p->coff = NULL;
+    p->tw2 = NULL;
+}
+
+/* tile_rows, slot_cap: the cross-term tile shape (gamma2 rows per tile,
+ * pairing slots per m-chunk); 0 takes the defaults (environment knobs, then
+ * the form's defaults) */
+static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot_cap)
+{
+    memset(p, 0, sizeof(*p));
+    p->A = *A;
+    p->tile_rows = tile_rows;
     const int L = A->L, K = L + 1, K2 = K * K, N = 2 * L + 1, Q = A->qnum, n = A->n;
     const int nb = A->nbeta, na = A->nalpha, ng = A->ngamma, zn = A->znum;
-    const int nrows = nb * ng;
-    const long nkeys = (long)zn * nb * ng;
-    if (nkeys > 0x3fffffffL || (long)nrows * Q > 0x7fffffffL) return FSX_ERR_ARG;
+    p->L = L; p->K = K; p->K2 = K2; p->N = N; p->Q = Q; p->n = n;
+    p->nb = nb; p->na = na; p->ng = ng; p->zn = zn;
+    p->nrows = nb * ng;
+    p->nkeys = (long)zn * nb * ng;
+    if (p->nkeys > 0x3fffffffL || (long)p->nrows * Q > 0x7fffffffL) return FSX_ERR_ARG;
     if ((long)L * na >= (1L << 24) || (long)L * ng >= (1L << 24)) return FSX_ERR_ARG;
-    const size_t dsz = (size_t)K * N * N;
-    const size_t spf = (size_t)3 * Q * K2;
-    const int mixed = (A->flags & FSX_MIXED_FP32) != 0;
+    p->dsz = (size_t)K * N * N;
+    p->spf = (size_t)3 * Q * K2;
+    p->mixed = (A->flags & FSX_MIXED_FP32) != 0;
+    p->dev_out = (A->flags & FSX_DEVICE_OUTPUT) != 0;
     /* half entry set (m >= 0) when both molecules' coefficients are
      * conjugate symmetric; FSX_CROSS_HALF=0 forces the full set */
-    const int half = env_int("FSX_CROSS_HALF", 1, 0, 1) &&
-                     spf_conj_symmetric(A->A_re, A->A_im, Q, L) &&
-                     spf_conj_symmetric(A->B_re, A->B_im, Q, L);
-    const int mlo = half ? 0 : -L;
+    p->half = env_int("FSX_CROSS_HALF", 1, 0, 1) && spf_conj_symmetric(A->A_re, A->A_im, Q, L) &&
+              spf_conj_symmetric(A->B_re, A->B_im, Q, L);
+    p->mlo = p->half ? 0 : -L;
     /* half set: the m = 0 entries of the receptor table and of U share
      * pairing slots two by two (FSX_CROSS_PACK=0 keeps one slot each) */
-    const int npk = half && env_int("FSX_CROSS_PACK", 1, 0, 1) ? K / 2 : 0;
-    const size_t es = mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
-    hipStream_t s = g_stream;
+    p->npk = p->half && env_int("FSX_CROSS_PACK", 1, 0, 1) ? K / 2 : 0;
+    p->es = p->mixed ? sizeof(cplx32) : sizeof(cplx);   /* table element size */
     /* m-chunks: one launch per chunk of entries, starting from one chunk
      * (FSX_CROSS_CHUNKS: the smallest count to try); more below while a chunk
      * is too large */
     int C = env_int("FSX_CROSS_CHUNKS", 1, 1, N);
-    const int Tn = tpk_sq(K);
-    const size_t nent = 8 * (size_t)K2 + 2;
-    int *ent = (int *)malloc(sizeof(int) * nent);
-    long *coff = (long *)malloc(sizeof(long) * (N + 1));
-    if (!ent || !coff) {
-        free(ent);
-        free(coff);
+    p->Tn = tpk_sq(K);
+    p->nent = 8 * (size_t)K2 + 2;
+    p->ent = (int *)malloc(sizeof(int) * p->nent);
+    p->coff = (long *)malloc(sizeof(long) * (N + 1));
+    p->tw2 = (double *)malloc(sizeof(double) * 2 * (size_t)na * N);
+    if (!p->ent || !p->coff || !p->tw2) {
+        plan_free_host(p);
         return FSX_ERR_NOMEM;
     }
+    int *ent = p->ent;
     /* more chunks until the largest fits: at most 4 entries per lane, and
      * for FP32 tables at most 128 pairing slots (the two-poses-per-wavefront
      * pairing); FSX_CROSS_CHUNK overrides the slot limit */
-    const int cmax = env_int("FSX_CROSS_CHUNK", mixed ? 2 * FSX_WAVE : 4 * FSX_WAVE, 64,
-                             4 * FSX_WAVE);
+    const int cmax = slot_cap > 0 ? slot_cap
+                                  : env_int("FSX_CROSS_CHUNK", p->mixed ? 2 * FSX_WAVE : 4 * FSX_WAVE,
+                                            64, 4 * FSX_WAVE);
     int ncmax, nsmax, ne;
     for (;;) {
-        ne = entry_tables(L, C, mlo, ent);
+        ne = entry_tables(L, C, p->mlo, ent);
         ncmax = nsmax = 0;
         for (int c = 0; c < C; c++) {
-            const int nc = ent[4 * K2 + c], ns = nc - (c == 0 ? npk : 0);
+            const int nc = ent[4 * K2 + c], ns = nc - (c == 0 ? p->npk : 0);
             if (nc > ncmax) ncmax = nc;
             if (ns > nsmax) nsmax = ns;
         }
@@ -1588,10 +1647,11 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
         C++;
     }
     if (ncmax > 4 * FSX_WAVE) {
-        free(ent);
-        free(coff);
+        plan_free_host(p);
         return FSX_ERR_ARG;
     }
+    p->C = C;
+    p->ne = ne;
     /* U-phase thread -> entry of each chunk: entries by increasing |m|
      * (decreasing row length K - |m|), in entry order within one |m| */
     {
@@ -1608,249 +1668,276 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     }
     /* twiddle e^{-i m a2 2pi/na} per (a2, m): the same table entries the
      * reference's exact transform uses, gathered by (m a2) mod na */
-    double *tw2 = (double *)malloc(sizeof(double) * 2 * (size_t)na * N);
-    if (!tw2) {
-        free(ent);
-        free(coff);
-        return FSX_ERR_NOMEM;
-    }
     for (int a2 = 0; a2 < na; a2++)
         for (int m = -L; m <= L; m++) {
             long k = ((long)m * a2) % na;
             if (k < 0) k += na;
             /* half set: the m > 0 terms stand for m and -m (an exact
              * factor 2) */
-            const double f = half && m > 0 ? 2.0 : 1.0;
-            tw2[2 * ((long)a2 * N + m + L)] = f * A->tw_a[2 * k];
-            tw2[2 * ((long)a2 * N + m + L) + 1] = f * A->tw_a[2 * k + 1];
+            const double f = p->half && m > 0 ? 2.0 : 1.0;
+            p->tw2[2 * ((long)a2 * N + m + L)] = f * A->tw_a[2 * k];
+            p->tw2[2 * ((long)a2 * N + m + L) + 1] = f * A->tw_a[2 * k + 1];
         }
     /* the mixed form reads them as FP32 pairs, in place */
-    if (A->flags & FSX_MIXED_FP32) {
-        float *f32 = (float *)tw2;
-        for (long i = 0; i < 2L * na * N; i++) f32[i] = (float)tw2[i];
+    if (p->mixed) {
+        float *f32 = (float *)p->tw2;
+        for (long i = 0; i < 2L * na * N; i++) f32[i] = (float)p->tw2[i];
     }
-    const int *csz = ent + 4 * K2, *cst = ent + 6 * K2;
-    coff[0] = 0;
-    for (int c = 0; c < C; c++) coff[c + 1] = coff[c] + (long)Q * nrows * 3 * csz[c];
-    int rc = FSX_OK;
-    hipEvent_t ev[8];
-    int nev_made = 0;   /* events created so far (destroyed on every exit) */
-#define RCF(x)                             \
-    do {                                   \
-        rc = (x);                          \
-        if (rc != FSX_OK) goto out;        \
-    } while (0)
-#define HIPF(x)                                                                       \
+    const int *csz = ent + 4 * K2;
+    p->coff[0] = 0;
+    for (int c = 0; c < C; c++) p->coff[c + 1] = p->coff[c] + (long)Q * p->nrows * 3 * csz[c];
+    /* buffer sizes */
+    const size_t es = p->es, spfe = (size_t)3 * Q * ne;   /* rotated-table entries per row */
+    size_t *z = p->sz;
+    z[P_BES] = sizeof(double) * zn * Q * N;
+    z[P_T] = es * zn * Q * (size_t)p->Tn;
+    z[P_RT] = z[P_BT] = z[P_RTMP] = es * (size_t)p->nrows * spfe;
+    z[P_A] = z[P_B] = sizeof(double) * 2 * p->spf;
+    z[P_DW] = sizeof(double) * p->dsz * nb;
+    z[P_DS] = sizeof(double) * (size_t)K * K2 * N;
+    z[P_TWA] = sizeof(double) * 2 * na;
+    z[P_TWG] = sizeof(double) * 2 * ng;
+    z[P_Q] = sizeof(double) * Q;
+    z[P_Z] = sizeof(double) * zn;
+    z[P_AM] = z[P_CINT] = sizeof(double) * 6 * Q;
+    z[P_KEYS] = sizeof(int) * n;
+    z[P_INFO] = sizeof(int2) * n;
+    z[P_COUNTS] = z[P_FILL] = sizeof(int) * p->nkeys;
+    z[P_OFFS] = sizeof(int) * (p->nkeys + 2);
+    z[P_RECS] = sizeof(int4) * n;
+    z[P_ACC] = sizeof(double) * 6 * Q * (size_t)n;
+    z[P_FITSC] = sizeof(double) * (size_t)(n > 0 ? n : 1);
+    z[P_ENT] = sizeof(int) * p->nent;
+    z[P_TW2] = sizeof(double) * 2 * (size_t)na * N;
+    z[P_COFF] = sizeof(long) * (C + 1);
+    z[P_NEXT] = sizeof(int);
+    z[P_TALLY] = 2 * sizeof(unsigned long long);
+    z[P_SRCT] = sizeof(cplx) * (size_t)Q * 3 * K2 * ng;
+    z[P_POSES] = (A->flags & FSX_DEVICE_POSES) ? 0 : sizeof(long long) * n;
+    z[P_SCORE] = z[P_C1] = z[P_C2] = p->dev_out ? 0 : sizeof(double) * n;
+    z[P_CROSSOUT] = (!p->dev_out && A->cross) ? sizeof(double) * 6 * Q * (size_t)n : 0;
+    /* the fit launch: as many lanes as are resident at once (the fit
+     * state lives in registers, one 256-thread block per CU) */
+    p->aq_lds = fit_aq_lds(Q);
+    p->lds_fit = fit_lds(Q);
+    p->kfit = p->aq_lds ? k_fit<1> : k_fit<0>;
+    int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
+    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, p->kfit, 256,
+                                                                      p->lds_fit) != hipSuccess ||
+                         fit_bpc < 1))
+        fit_bpc = 1;
+    p->lanes = (long)g_ncu * 256 * fit_bpc;
+    const int fit_blocks = env_int("FSX_FIT_BLOCKS", 0, 0, 1 << 20);
+    if (fit_blocks > 0) p->lanes = 256L * fit_blocks;
+    if (p->lanes > n) p->lanes = ((long)n + 255) / 256 * 256;
+    return FSX_OK;
+}
+
+/* Device pointers of the plan from get(buffer index, bytes, ctx); the
+ * device-resident inputs and outputs of the call are used in place. */
+static int plan_bind(fsx_plan *p, int (*get)(int, size_t, void *, void **), void *ctx)
+{
+    for (int i = 0; i < P_NBUF; i++) {
+        p->d[i] = NULL;
+        if (p->sz[i] == 0) continue;
+        RC(get(i, p->sz[i], ctx, &p->d[i]));
+    }
+    const fsx_hip_args *A = &p->A;
+    p->dposes = (A->flags & FSX_DEVICE_POSES) ? (const long long *)A->poses
+                                              : (const long long *)p->d[P_POSES];
+    if (p->dev_out) {
+        p->dscore = A->score; p->dc1 = A->c1; p->dc2 = A->c2; p->dcross = A->cross;
+    } else {
+        p->dscore = (double *)p->d[P_SCORE];
+        p->dc1 = (double *)p->d[P_C1];
+        p->dc2 = (double *)p->d[P_C2];
+        p->dcross = (double *)p->d[P_CROSSOUT];
+    }
+    return FSX_OK;
+}
+
+#define HIPR(x)                                                                       \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
         if (e_ != hipSuccess) {                                                       \
             fprintf(stderr, "[fsx] %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
                     __FILE__, __LINE__);                                              \
-            rc = FSX_ERR_HIP;                                                         \
-            goto out;                                                                 \
+            return FSX_ERR_HIP;                                                       \
         }                                                                             \
     } while (0)
-    {
-    RCF(ensure(B_BES, sizeof(double) * zn * Q * N));
-    RCF(ensure(B_T, es * zn * Q * (size_t)Tn));
-    const size_t spfe = (size_t)3 * Q * ne;   /* rotated-table entries per row */
-    RCF(ensure(B_RT, es * (size_t)nrows * spfe));
-    RCF(ensure(B_BT, es * (size_t)nrows * spfe));
-    RCF(ensure(B_RTMP, es * (size_t)nrows * spfe));
-    RCF(ensure(B_A, sizeof(double) * 2 * spf));
-    RCF(ensure(B_B, sizeof(double) * 2 * spf));
-    RCF(ensure(B_DW, sizeof(double) * dsz * nb));
-    RCF(ensure(B_DS, sizeof(double) * (size_t)K * K2 * N));
-    RCF(ensure(B_TWA, sizeof(double) * 2 * na));
-    RCF(ensure(B_TWG, sizeof(double) * 2 * ng));
-    RCF(ensure(B_Q, sizeof(double) * Q));
-    RCF(ensure(B_Z, sizeof(double) * zn));
-    RCF(ensure(B_AM, sizeof(double) * 6 * Q));
-    RCF(ensure(B_CINT, sizeof(double) * 6 * Q));
-    RCF(ensure(B_KEYS, sizeof(int) * n));
-    RCF(ensure(B_INFO, sizeof(int2) * n));
-    RCF(ensure(B_COUNTS, sizeof(int) * nkeys));
-    RCF(ensure(B_OFFS, sizeof(int) * (nkeys + 2)));
-    RCF(ensure(B_FILL, sizeof(int) * nkeys));
-    RCF(ensure(B_RECS, sizeof(int4) * n));
-    RCF(ensure(B_ACC, sizeof(double) * 6 * Q * (size_t)n));
-    RCF(ensure(B_FITSC, sizeof(double) * (size_t)(n > 0 ? n : 1)));
-    RCF(ensure(B_ENT, sizeof(int) * nent));
-    RCF(ensure(B_TW2, sizeof(double) * 2 * (size_t)na * N));
-    RCF(ensure(B_COFF, sizeof(long) * (C + 1)));
-    RCF(ensure(B_NEXT, sizeof(int)));
-    RCF(ensure(B_TALLY, 2 * sizeof(unsigned long long)));
-    RCF(ensure(B_SRCT, sizeof(cplx) * (size_t)Q * 3 * K2 * ng));
-    double *dA = (double *)g_buf[B_A], *dB = (double *)g_buf[B_B];
-    HIPF(hipMemcpyAsync(dA, A->A_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(dA + spf, A->A_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(dB, A->B_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(dB + spf, A->B_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
-    if (!(A->dw_key && g_key[B_DW] == A->dw_key)) {
-        HIPF(hipMemcpyAsync(g_buf[B_DW], A->dW, sizeof(double) * dsz * nb, hipMemcpyHostToDevice, s));
-        g_key[B_DW] = A->dw_key;
-    }
-    if (!(A->ds_key && g_key[B_DS] == A->ds_key)) {
-        HIPF(hipMemcpyAsync(g_buf[B_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
-        g_key[B_DS] = A->ds_key;
-    }
-    HIPF(hipMemcpyAsync(g_buf[B_TWA], A->tw_a, sizeof(double) * 2 * na, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_TWG], A->tw_g, sizeof(double) * 2 * ng, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_Q], A->qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_Z], A->zvals, sizeof(double) * zn, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_AM], A->a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_CINT], A->cint, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_ENT], ent, sizeof(int) * nent, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_TW2], tw2, sizeof(double) * 2 * (size_t)na * N, hipMemcpyHostToDevice, s));
-    HIPF(hipMemcpyAsync(g_buf[B_COFF], coff, sizeof(long) * (C + 1), hipMemcpyHostToDevice, s));
 
-    const long long *dposes;
-    if (A->flags & FSX_DEVICE_POSES) {
-        dposes = (const long long *)A->poses;
-    } else {
-        RCF(ensure(B_POSES, sizeof(long long) * n));
-        HIPF(hipMemcpyAsync(g_buf[B_POSES], A->poses, sizeof(long long) * n, hipMemcpyHostToDevice, s));
-        dposes = (const long long *)g_buf[B_POSES];
-    }
-    double *dscore, *dc1, *dc2, *dcross = NULL;
-    int dev_out = (A->flags & FSX_DEVICE_OUTPUT) != 0;
-    if (dev_out) {
-        dscore = A->score; dc1 = A->c1; dc2 = A->c2; dcross = A->cross;
-    } else {
+/* Inputs and tables to the plan's buffers.  up_dw / up_ds: whether the
+ * Wigner-d and d-symbol tables must be copied (the buffers may already
+ * hold them). */
+static int plan_upload(fsx_plan *p, hipStream_t s, int up_dw, int up_ds)
+{
+    const fsx_hip_args *A = &p->A;
+    const size_t spf = p->spf;
+    const int Q = p->Q, n = p->n, K = p->K, K2 = p->K2, N = p->N;
+    double *dA = (double *)p->d[P_A], *dB = (double *)p->d[P_B];
+    HIPR(hipMemcpyAsync(dA, A->A_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(dA + spf, A->A_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(dB, A->B_re, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(dB + spf, A->B_im, sizeof(double) * spf, hipMemcpyHostToDevice, s));
+    if (up_dw) HIPR(hipMemcpyAsync(p->d[P_DW], A->dW, p->sz[P_DW], hipMemcpyHostToDevice, s));
+    if (up_ds)
+        HIPR(hipMemcpyAsync(p->d[P_DS], A->dsymb, sizeof(double) * K * K2 * N, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_TWA], A->tw_a, p->sz[P_TWA], hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_TWG], A->tw_g, p->sz[P_TWG], hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_Q], A->qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_Z], A->zvals, sizeof(double) * p->zn, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_AM], A->a, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_CINT], A->cint, sizeof(double) * 6 * Q, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_ENT], p->ent, sizeof(int) * p->nent, hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_TW2], p->tw2, p->sz[P_TW2], hipMemcpyHostToDevice, s));
+    HIPR(hipMemcpyAsync(p->d[P_COFF], p->coff, sizeof(long) * (p->C + 1), hipMemcpyHostToDevice, s));
+    if (!(A->flags & FSX_DEVICE_POSES))
+        HIPR(hipMemcpyAsync(p->d[P_POSES], A->poses, sizeof(long long) * n, hipMemcpyHostToDevice, s));
+    if (!p->dev_out) {
         /* device mirrors start from the caller's values so that poses the
          * reference leaves untouched stay untouched */
-        RCF(ensure(B_SCORE, sizeof(double) * n));
-        RCF(ensure(B_C1, sizeof(double) * n));
-        RCF(ensure(B_C2, sizeof(double) * n));
-        dscore = (double *)g_buf[B_SCORE]; dc1 = (double *)g_buf[B_C1]; dc2 = (double *)g_buf[B_C2];
-        HIPF(hipMemcpyAsync(dscore, A->score, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        HIPF(hipMemcpyAsync(dc1, A->c1, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        HIPF(hipMemcpyAsync(dc2, A->c2, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        if (A->cross) {
-            RCF(ensure(B_CROSSOUT, sizeof(double) * 6 * Q * (size_t)n));
-            dcross = (double *)g_buf[B_CROSSOUT];
-            HIPF(hipMemcpyAsync(dcross, A->cross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyHostToDevice, s));
-        }
+        HIPR(hipMemcpyAsync(p->dscore, A->score, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPR(hipMemcpyAsync(p->dc1, A->c1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPR(hipMemcpyAsync(p->dc2, A->c2, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (A->cross)
+            HIPR(hipMemcpyAsync(p->dcross, A->cross, sizeof(double) * 6 * Q * (size_t)n,
+                                hipMemcpyHostToDevice, s));
     }
+    return FSX_OK;
+}
 
-    for (; nev_made < 8; nev_made++) HIPF(hipEventCreate(&ev[nev_made]));
-    HIPF(hipEventRecord(ev[0], s));
+/* Every stage of the plan on stream s, no waiting; ev (may be NULL): seven
+ * events recorded at the stage boundaries (after the tables, rotations,
+ * bucketing, cross terms, fit preparation and fit). */
+static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_after_cross = 0)
+{
+    const fsx_hip_args *A = &p->A;
+    const int L = p->L, K = p->K, K2 = p->K2, N = p->N, Q = p->Q, n = p->n;
+    const int nb = p->nb, na = p->na, ng = p->ng, zn = p->zn, nrows = p->nrows, C = p->C;
+    const long nkeys = p->nkeys;
+    const int *csz = p->ent + 4 * K2, *cst = p->ent + 6 * K2;
+    const long *coff = p->coff;
+    void **g = p->d;
+    (void)na;
     /* translation tables */
-    long nbes = (long)zn * Q * N;
-    hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g_buf[B_BES],
-                       (const double *)g_buf[B_Z], (const double *)g_buf[B_Q], zn, Q, N);
-    long nt = (long)zn * Q * tpk_sq(K);
-    if (mixed) launch_tmat<cplx32>(s, g_buf[B_T], nt, zn, Q, L);
-    else launch_tmat<cplx>(s, g_buf[B_T], nt, zn, Q, L);
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[1], s));
+    const long nbes = (long)zn * Q * N;
+    hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g[P_BES],
+                       (const double *)g[P_Z], (const double *)g[P_Q], zn, Q, N);
+    const long nt = (long)zn * Q * tpk_sq(K);
+    if (p->mixed)
+        hipLaunchKernelGGL(k_tmat<cplx32>, dim3(nblk(nt, 256)), dim3(256), 0, s, (float *)g[P_T],
+                           (const double *)g[P_BES], (const double *)g[P_DS], zn, Q, L);
+    else
+        hipLaunchKernelGGL(k_tmat<cplx>, dim3(nblk(nt, 256)), dim3(256), 0, s, (double *)g[P_T],
+                           (const double *)g[P_BES], (const double *)g[P_DS], zn, Q, L);
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[1], s));
     /* rotated receptor and ligand coefficients, m-chunked */
     {
-        long ntw = (long)Q * 3 * K2 * ng;
-        cplx *srct = (cplx *)g_buf[B_SRCT];
+        const long ntw = (long)Q * 3 * K2 * ng;
+        cplx *srct = (cplx *)g[P_SRCT];
+        const double *dA = (const double *)g[P_A], *dB = (const double *)g[P_B];
+        const size_t spf = p->spf;
         /* gamma chunk: the staged rows (2L+1) x gc complex stay within 48 KB */
         int gc = ng < FSX_WAVE ? ng : FSX_WAVE;
         while (gc > 1 && (size_t)N * gc * sizeof(cplx) > 48 * 1024) gc /= 2;
         const long nrot = (long)Q * 3 * K * ((nb + FSX_ROT_BC - 1) / FSX_ROT_BC) * ((ng + gc - 1) / gc);
         const size_t lds_rot = (size_t)N * gc * sizeof(cplx) + (size_t)FSX_ROT_BC * N * N * sizeof(double);
-        hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
-                           (const double *)dA, (const double *)(dA + spf),
-                           (const double *)g_buf[B_TWG], ng, Q, L, 0);
-        if (mixed)
-            launch_rotate<cplx32>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, cst, coff, mlo, npk, 1);
-        else
-            launch_rotate<cplx>(s, g_buf[B_RT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, cst, coff, mlo, npk, 1);
-        hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct,
-                           (const double *)dB, (const double *)(dB + spf),
-                           (const double *)g_buf[B_TWG], ng, Q, L, 1);
-        if (mixed)
-            launch_rotate<cplx32>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q,
-                                  L, gc, C, csz, cst, coff, mlo, 0, -1);
-        else
-            launch_rotate<cplx>(s, g_buf[B_BT], g_buf[B_RTMP], srct, nrot, lds_rot, nb, ng, Q, L,
-                                gc, C, csz, cst, coff, mlo, 0, -1);
+        for (int side = 0; side < 2; side++) {
+            const double *src = side == 0 ? dA : dB;
+            void *out = side == 0 ? g[P_RT] : g[P_BT];
+            hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct, src,
+                               src + spf, (const double *)g[P_TWG], ng, Q, L, side);
+            const int npk = side == 0 ? p->npk : 0, lrot = side == 0 ? 1 : -1;
+            if (p->mixed)
+                launch_rotate<cplx32>(s, out, g[P_RTMP], srct, nrot, lds_rot, nb, ng, Q, L, gc, C,
+                                      csz, cst, coff, p->mlo, npk, lrot, g);
+            else
+                launch_rotate<cplx>(s, out, g[P_RTMP], srct, nrot, lds_rot, nb, ng, Q, L, gc, C,
+                                    csz, cst, coff, p->mlo, npk, lrot, g);
+        }
     }
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[2], s));
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[2], s));
     /* bucket poses by (z, beta2, gamma2) */
-    int *counts = (int *)g_buf[B_COUNTS], *offs = (int *)g_buf[B_OFFS], *fill = (int *)g_buf[B_FILL];
+    int *counts = (int *)g[P_COUNTS], *offs = (int *)g[P_OFFS], *fill = (int *)g[P_FILL];
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, counts, nkeys, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, fill, nkeys, 0);
-    hipLaunchKernelGGL(k_count, dim3(nblk(n, 256)), dim3(256), 0, s, dposes, n, zn, nb, na, ng,
-                       (int *)g_buf[B_KEYS], (int2 *)g_buf[B_INFO], counts);
+    hipLaunchKernelGGL(k_count, dim3(nblk(n, 256)), dim3(256), 0, s, p->dposes, n, zn, nb, p->na,
+                       ng, (int *)g[P_KEYS], (int2 *)g[P_INFO], counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const int *)counts, (int)nkeys, offs);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, (const int *)g_buf[B_KEYS],
-                       (const int2 *)g_buf[B_INFO], n, (const int *)offs, fill,
-                       (int4 *)g_buf[B_RECS]);
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[3], s));
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, (const int *)g[P_KEYS],
+                       (const int2 *)g[P_INFO], n, (const int *)offs, fill, (int4 *)g[P_RECS]);
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[3], s));
     /* per-pose cross terms, one launch per m-chunk, accumulated in order */
-    long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
-    if (mixed) launch_cross<cplx32>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk, ng, zn);
-    else launch_cross<cplx>(s, C, csz, cst, coff, Tn, grid, nrows, Q, L, nb, npk, ng, zn);
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[4], s));
-    /* per-pose fits over a pose queue: as many lanes as are resident at once
-     * (the fit state lives in registers, one 256-thread block per CU) */
-    const int aq_lds = fit_aq_lds(Q);
-    const size_t lds_fit = fit_lds(Q);
-    int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
-    void (*kfit)(const double *, const int4 *, const int *, int, const double *, const double *,
-                 double, double, double, double, double, double *, double *, double *, int *,
-                 unsigned long long *) = aq_lds ? k_fit<1> : k_fit<0>;
-    if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, kfit, 256, lds_fit)
-                             != hipSuccess || fit_bpc < 1))
-        fit_bpc = 1;
-    long lanes = (long)g_ncu * 256 * fit_bpc;
-    const int fit_blocks = env_int("FSX_FIT_BLOCKS", 0, 0, 1 << 20);
-    if (fit_blocks > 0) lanes = 256L * fit_blocks;
-    if (lanes > n) lanes = ((long)n + 255) / 256 * 256;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_NEXT], 1L, (int)lanes);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g_buf[B_TALLY], 4L, 0);
+    const long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
+    if (p->mixed)
+        launch_cross<cplx32>(s, C, csz, cst, coff, p->Tn, grid, nrows, Q, L, nb, p->npk, ng, zn, g,
+                             p->tile_rows);
+    else
+        launch_cross<cplx>(s, C, csz, cst, coff, p->Tn, grid, nrows, Q, L, nb, p->npk, ng, zn, g,
+                           p->tile_rows);
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[4], s));
+    if (stop_after_cross) return FSX_OK;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_NEXT], 1L, (int)p->lanes);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_TALLY], 4L, 0);
     /* rescaled profiles in place of the pose parts (bounded by n slots;
      * the kernels read the number of valid slots on the device) */
-    hipLaunchKernelGGL(k_fit_scale, dim3(nblk(n, 256)), dim3(256), 0, s,
-                       (const double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
-                       (const int *)(offs + nkeys), Q, A->peak, (double *)g_buf[B_FITSC]);
+    hipLaunchKernelGGL(k_fit_scale, dim3(nblk(n, 256)), dim3(256), 0, s, (const double *)g[P_ACC],
+                       (const double *)g[P_CINT], (const int *)(offs + nkeys), Q, A->peak,
+                       (double *)g[P_FITSC]);
     hipLaunchKernelGGL(k_fit_prep, dim3(nblk((long)n * Q, 256)), dim3(256), 0, s,
-                       (double *)g_buf[B_ACC], (const double *)g_buf[B_CINT],
-                       (const int4 *)g_buf[B_RECS], (const int *)(offs + nkeys), Q,
-                       (const double *)g_buf[B_FITSC], dcross);
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
-    hipLaunchKernelGGL(kfit, dim3((unsigned)(lanes / 256)), dim3(256), lds_fit, s,
-                       (const double *)g_buf[B_ACC], (const int4 *)g_buf[B_RECS],
-                       (const int *)(offs + nkeys), Q, (const double *)g_buf[B_Q],
-                       (const double *)g_buf[B_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo,
-                       A->c2_hi, dscore, dc1, dc2, (int *)g_buf[B_NEXT],
-                       (unsigned long long *)g_buf[B_TALLY]);
-    HIPF(hipGetLastError());
-    HIPF(hipEventRecord(ev[6], s));
-    if (!dev_out) {
-        HIPF(hipMemcpyAsync(A->score, dscore, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        HIPF(hipMemcpyAsync(A->c1, dc1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        HIPF(hipMemcpyAsync(A->c2, dc2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                       (double *)g[P_ACC], (const double *)g[P_CINT], (const int4 *)g[P_RECS],
+                       (const int *)(offs + nkeys), Q, (const double *)g[P_FITSC], p->dcross);
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
+    hipLaunchKernelGGL(p->kfit, dim3((unsigned)(p->lanes / 256)), dim3(256), p->lds_fit, s,
+                       (const double *)g[P_ACC], (const int4 *)g[P_RECS],
+                       (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
+                       (const double *)g[P_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo, A->c2_hi,
+                       p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT], (unsigned long long *)g[P_TALLY]);
+    HIPR(hipGetLastError());
+    if (ev) HIPR(hipEventRecord(ev[6], s));
+    return FSX_OK;
+}
+
+/* Results to the host (host outputs) and the call's counters to tail /
+ * tally, on stream s. */
+static int plan_download(fsx_plan *p, hipStream_t s, int *tail, unsigned long long *tally)
+{
+    const fsx_hip_args *A = &p->A;
+    const int n = p->n, Q = p->Q;
+    if (!p->dev_out) {
+        HIPR(hipMemcpyAsync(A->score, p->dscore, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPR(hipMemcpyAsync(A->c1, p->dc1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPR(hipMemcpyAsync(A->c2, p->dc2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         if (A->cross)
-            HIPF(hipMemcpyAsync(A->cross, dcross, sizeof(double) * 6 * Q * (size_t)n, hipMemcpyDeviceToHost, s));
+            HIPR(hipMemcpyAsync(A->cross, p->dcross, sizeof(double) * 6 * Q * (size_t)n,
+                                hipMemcpyDeviceToHost, s));
     }
-    int tail[2] = {0, 0};
-    unsigned long long tally[2] = {0, 0};
-    HIPF(hipMemcpyAsync(tail, offs + nkeys, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-    HIPF(hipMemcpyAsync(tally, g_buf[B_TALLY], sizeof(tally), hipMemcpyDeviceToHost, s));
-    HIPF(hipEventRecord(ev[7], s));
-    HIPF(hipStreamSynchronize(s));
-    float ms[7];
-    for (int i = 0; i < 7; i++) HIPF(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    HIPR(hipMemcpyAsync(tail, (int *)p->d[P_OFFS] + p->nkeys, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIPR(hipMemcpyAsync(tally, p->d[P_TALLY], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    return FSX_OK;
+}
+
+/* The statistics of a finished plan: counters, algorithmic work and (ms not
+ * NULL) the six stage times. */
+static void plan_stats(const fsx_plan *p, fsx_stats *st, const float *ms, const int *tail,
+                       const unsigned long long *tally)
+{
+    const int K = p->K, K2 = p->K2, Q = p->Q, ne = p->ne, npk = p->npk, C = p->C;
     const int nvalid = tail[0];
     const long items = (long)Q * tail[1];
-    st->ms_bessel_t = ms[0];
-    st->ms_rotate = ms[1];
-    st->ms_group = ms[2];
-    st->ms_cross = ms[3];
-    st->ms_fit_prep = ms[4];
-    st->ms_fit = ms[5];
+    if (ms) {
+        st->ms_bessel_t = ms[0];
+        st->ms_rotate = ms[1];
+        st->ms_group = ms[2];
+        st->ms_cross = ms[3];
+        st->ms_fit_prep = ms[4];
+        st->ms_fit = ms[5];
+    }
     st->n_scored = nvalid;
     st->fit_evals = (long)tally[0];
     st->n_capped = (int)tally[1];
@@ -1861,23 +1948,65 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
      * terms per item) and 54 per (pose, q, pairing slot); the npk packed
      * slots carry two m = 0 entries whose other halves are zero */
     double tpair = 0.0;
-    for (int e = 0; e < ne; e++) tpair += K - (ent[K2 + e] < 0 ? -ent[K2 + e] : ent[K2 + e]);
+    for (int e = 0; e < ne; e++) tpair += K - (p->ent[K2 + e] < 0 ? -p->ent[K2 + e] : p->ent[K2 + e]);
     const double nslot = (double)(ne - npk);
     st->cross_flops = (double)items * 3.0 * 8.0 * tpair + (double)nvalid * Q * nslot * 54.0;
     /* algorithmic bytes: per pose and q one receptor row (3 parts, one
      * value per slot) and the six-term accumulator (read and written by
      * every chunk after the first); per item one ligand row */
-    st->cross_bytes = (double)nvalid * Q * (3.0 * nslot * (double)es + (2.0 * C - 1.0) * 48.0)
-                    + (double)items * 3.0 * ne * (double)es;
+    st->cross_bytes = (double)nvalid * Q * (3.0 * nslot * (double)p->es + (2.0 * C - 1.0) * 48.0)
+                    + (double)items * 3.0 * ne * (double)p->es;
     st->cross_entries = ne;
+}
+
+static_assert(G_T == P_T && G_RT == P_RT && G_BT == P_BT && G_DW == P_DW && G_COUNTS == P_COUNTS &&
+                  G_OFFS == P_OFFS && G_RECS == P_RECS && G_ACC == P_ACC && G_ENT == P_ENT &&
+                  G_TW2 == P_TW2 && G_COFF == P_COFF,
+              "plan buffer indices of the launch helpers");
+
+/* single calls: the process-wide buffer slots */
+static int get_slot(int i, size_t bytes, void *ctx, void **out)
+{
+    (void)ctx;
+    RC(ensure(k_plan_slot[i], bytes));
+    *out = g_buf[k_plan_slot[i]];
+    return FSX_OK;
+}
+
+static int run_locked(const fsx_hip_args *A, fsx_stats *st)
+{
+    RC(init_once());
+    fsx_plan p;
+    RC(plan_init(&p, A, 0, 0));
+    hipStream_t s = g_stream;
+    hipEvent_t ev[7];
+    int nev_made = 0, rc = FSX_OK;
+    int tail[2] = {0, 0};
+    unsigned long long tally[2] = {0, 0};
+    float ms[6];
+    rc = plan_bind(&p, get_slot, NULL);
+    if (rc == FSX_OK) {
+        /* the Wigner-d and d-symbol tables are not copied again when the
+         * buffer already holds the same table (content keys) */
+        const int up_dw = !(A->dw_key && g_key[B_DW] == A->dw_key);
+        const int up_ds = !(A->ds_key && g_key[B_DS] == A->ds_key);
+        rc = plan_upload(&p, s, up_dw, up_ds);
+        if (rc == FSX_OK) {
+            g_key[B_DW] = A->dw_key;
+            g_key[B_DS] = A->ds_key;
+        }
     }
-out:
+    for (; rc == FSX_OK && nev_made < 7; nev_made++)
+        if (hipEventCreate(&ev[nev_made]) != hipSuccess) rc = FSX_ERR_HIP;
+    if (rc == FSX_OK && hipEventRecord(ev[0], s) != hipSuccess) rc = FSX_ERR_HIP;
+    if (rc == FSX_OK) rc = plan_enqueue(&p, s, ev);
+    if (rc == FSX_OK) rc = plan_download(&p, s, tail, tally);
+    if (rc == FSX_OK && hipStreamSynchronize(s) != hipSuccess) rc = FSX_ERR_HIP;
+    for (int i = 0; rc == FSX_OK && i < 6; i++)
+        if (hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]) != hipSuccess) rc = FSX_ERR_HIP;
+    if (rc == FSX_OK) plan_stats(&p, st, ms, tail, tally);
     for (int i = 0; i < nev_made; i++) (void)hipEventDestroy(ev[i]);
-    free(ent);
-    free(coff);
-    free(tw2);
-#undef RCF
-#undef HIPF
+    plan_free_host(&p);
     return rc;
 }
 
@@ -1885,6 +2014,352 @@ extern "C" int fsx_hip_run(const fsx_hip_args *A, fsx_stats *st)
 {
     pthread_mutex_lock(&g_mtx);
     int rc = run_locked(A, st);
+    pthread_mutex_unlock(&g_mtx);
+    return rc;
+}
+
+/* ---- batches: several complexes in one call --------------------------------
+ * Each complex gets its own plan and buffer set.  The inputs are uploaded on
+ * the library's stream; then every complex's stages are launched on one of
+ * FSX_BSTREAMS streams (complex i on stream i mod FSX_BSTREAMS, forked from
+ * and joined back into the library's stream), so that the stages of small
+ * complexes run side by side.  That launch sequence is captured once into a
+ * HIP graph and replayed by one graph launch on later calls whose launch
+ * parameters (sizes, tile shapes, scalars, buffer addresses) are all the same.
+ * The cross-term tile shape per expansion-order bucket and form comes from a
+ * tuner that times the candidates once on the first complex of the bucket. */
+#define FSX_BSTREAMS 4
+struct fsx_bset {
+    void *d[P_NBUF];
+    size_t cap[P_NBUF];
+    unsigned long long dw_key, ds_key;   /* tables held by d[P_DW], d[P_DS] */
+};
+static fsx_bset *g_bat = NULL;
+static int g_nbat = 0;
+/* intermediates (written and read by the stages of one complex only) are
+ * shared by the complexes of one stream, which run one after another;
+ * inputs uploaded before the stages run, outputs and the counters read
+ * back afterwards are per complex */
+static fsx_bset g_bstr[FSX_BSTREAMS];
+static inline int per_stream_buffer(int i)
+{
+    switch (i) {
+    case P_BES: case P_T: case P_RT: case P_BT: case P_RTMP: case P_KEYS: case P_INFO:
+    case P_COUNTS: case P_FILL: case P_RECS: case P_ACC: case P_FITSC: case P_NEXT: case P_SRCT:
+        return 1;
+    default:
+        return 0;
+    }
+}
+struct fsx_bctx {
+    fsx_bset *cplx, *strm;
+};
+static hipStream_t g_bs[FSX_BSTREAMS];
+static hipEvent_t g_bfork, g_bjoin[FSX_BSTREAMS];
+static int g_binit = 0;
+static hipGraph_t g_graph = NULL;
+static hipGraphExec_t g_gexec = NULL;
+static void *g_gkey = NULL;
+static size_t g_gkey_len = 0;
+
+/* the launch parameters a captured graph depends on, per complex */
+struct fsx_gkey {
+    int L, nb, na, ng, Q, zn, n, flags, half, npk, C, tile_rows, aq_lds;
+    long lanes;
+    double mult, peak, c1_lo, c1_hi, c2_lo, c2_hi;
+    void *d[P_NBUF];
+    const void *dposes;
+    void *dscore, *dc1, *dc2, *dcross;
+};
+
+static int batch_init(void)
+{
+    if (g_binit) return FSX_OK;
+    for (int i = 0; i < FSX_BSTREAMS; i++) {
+        HIPCHK(hipStreamCreateWithFlags(&g_bs[i], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g_bjoin[i], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&g_bfork, hipEventDisableTiming));
+    g_binit = 1;
+    return FSX_OK;
+}
+
+static void graph_drop(void)
+{
+    if (g_gexec) (void)hipGraphExecDestroy(g_gexec);
+    if (g_graph) (void)hipGraphDestroy(g_graph);
+    g_gexec = NULL;
+    g_graph = NULL;
+    free(g_gkey);
+    g_gkey = NULL;
+    g_gkey_len = 0;
+}
+
+static int get_bat(int i, size_t bytes, void *ctx, void **out)
+{
+    const fsx_bctx *c = (const fsx_bctx *)ctx;
+    fsx_bset *b = per_stream_buffer(i) ? c->strm : c->cplx;
+    if (b->cap[i] < bytes) {
+        if (b->d[i]) HIPCHK(hipFree(b->d[i]));
+        b->d[i] = NULL;
+        b->cap[i] = 0;
+        if (i == P_DW) b->dw_key = 0;
+        if (i == P_DS) b->ds_key = 0;
+        const size_t cap = bytes + bytes / 8;
+        if (hipMalloc(&b->d[i], cap) != hipSuccess) {
+            b->d[i] = NULL;
+            return FSX_ERR_NOMEM;
+        }
+        b->cap[i] = cap;
+    }
+    *out = b->d[i];
+    return FSX_OK;
+}
+
+
+/* ---- tile-shape tuner ----
+ * Per expansion-order bucket (L <= 10, 15, 20, 25, 30, 40) and form, the
+ * gamma2 rows per cross-term tile with the shortest cross-term stage on the
+ * first complex of that bucket; FSX_TUNE_FILE names a text file the table is
+ * read from and appended to ("bucket mixed rows slots ms" lines, slots 0:
+ * the form's default pairing-slot cap). */
+#define FSX_TUNE_BUCKETS 6
+static int g_tune[FSX_TUNE_BUCKETS][2][2];   /* rows, slots; 0: not tuned */
+static int g_tune_loaded = 0;
+static void batch_release(void)
+{
+    graph_drop();
+    memset(g_tune, 0, sizeof(g_tune));
+    g_tune_loaded = 0;
+    for (int k = 0; k < g_nbat; k++)
+        for (int i = 0; i < P_NBUF; i++)
+            if (g_bat[k].d[i]) (void)hipFree(g_bat[k].d[i]);
+    for (int k = 0; k < FSX_BSTREAMS; k++)
+        for (int i = 0; i < P_NBUF; i++)
+            if (g_bstr[k].d[i]) (void)hipFree(g_bstr[k].d[i]);
+    memset(g_bstr, 0, sizeof(g_bstr));
+    free(g_bat);
+    g_bat = NULL;
+    g_nbat = 0;
+}
+
+static inline int tune_bucket(int L)
+{
+    return L <= 10 ? 0 : L <= 15 ? 1 : L <= 20 ? 2 : L <= 25 ? 3 : L <= 30 ? 4 : 5;
+}
+
+static void tune_load(void)
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = 1;
+    const char *path = getenv("FSX_TUNE_FILE");
+    if (!path || !*path) return;
+    FILE *f = fopen(path, "r");
+    if (!f) return;
+    int b, m, r, c;
+    double ms;
+    while (fscanf(f, "%d %d %d %d %lf", &b, &m, &r, &c, &ms) == 5)
+        if (b >= 0 && b < FSX_TUNE_BUCKETS && m >= 0 && m <= 1 && r >= 1 && r <= 4 &&
+            (c == 0 || (c >= 64 && c <= 4 * FSX_WAVE))) {
+            g_tune[b][m][0] = r;
+            g_tune[b][m][1] = c;
+        }
+    fclose(f);
+}
+
+static void tune_save(int b, int m, int r, int c, double ms)
+{
+    const char *path = getenv("FSX_TUNE_FILE");
+    if (!path || !*path) return;
+    FILE *f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%d %d %d %d %.4f\n", b, m, r, c, ms);
+    fclose(f);
+}
+
+/* Time the cross-term stage of complex A for each candidate shape in the
+ * process-wide buffers and record the fastest for its bucket. */
+static int tune_one(const fsx_hip_args *A)
+{
+    const int mixed = (A->flags & FSX_MIXED_FP32) != 0, b = tune_bucket(A->L);
+    /* gamma2 rows per tile; the pairing-slot cap stays the form's default,
+     * since it sets the m-chunks and with them the order of the FP32 sums
+     * (rows change no arithmetic: a tuned batch scores every pose exactly as
+     * a single call does) */
+    static const int cand32[][2] = {{4, 0}, {2, 0}, {1, 0}};
+    static const int cand64[][2] = {{2, 0}, {1, 0}, {4, 0}};
+    const int nc = 3;
+    hipEvent_t ev[7];
+    int nev = 0, rc = FSX_OK, best = -1;
+    float best_ms = 0.f;
+    for (; nev < 7; nev++)
+        if (hipEventCreate(&ev[nev]) != hipSuccess) {
+            rc = FSX_ERR_HIP;
+            break;
+        }
+    for (int k = 0; rc == FSX_OK && k < nc; k++) {
+        const int rows = mixed ? cand32[k][0] : cand64[k][0];
+        const int slots = mixed ? cand32[k][1] : cand64[k][1];
+        fsx_plan p;
+        rc = plan_init(&p, A, rows, slots);
+        if (rc != FSX_OK) break;
+        rc = plan_bind(&p, get_slot, NULL);
+        if (rc == FSX_OK) rc = plan_upload(&p, g_stream, 1, 1);
+        if (rc == FSX_OK) {
+            g_key[B_DW] = 0;   /* the buffers now hold this complex's tables */
+            g_key[B_DS] = 0;
+        }
+        /* twice: the first run warms the caches and code */
+        float ms = 0.f;
+        for (int rep = 0; rc == FSX_OK && rep < 2; rep++) {
+            if (hipEventRecord(ev[0], g_stream) != hipSuccess) rc = FSX_ERR_HIP;
+            if (rc == FSX_OK) rc = plan_enqueue(&p, g_stream, ev, 1);
+            if (rc == FSX_OK && hipStreamSynchronize(g_stream) != hipSuccess) rc = FSX_ERR_HIP;
+            if (rc == FSX_OK && hipEventElapsedTime(&ms, ev[3], ev[4]) != hipSuccess) rc = FSX_ERR_HIP;
+        }
+        plan_free_host(&p);
+        if (rc == FSX_OK && (best < 0 || ms < best_ms)) {
+            best = k;
+            best_ms = ms;
+        }
+    }
+    for (int i = 0; i < nev; i++) (void)hipEventDestroy(ev[i]);
+    if (rc == FSX_OK && best >= 0) {
+        g_tune[b][mixed][0] = mixed ? cand32[best][0] : cand64[best][0];
+        g_tune[b][mixed][1] = mixed ? cand32[best][1] : cand64[best][1];
+        tune_save(b, mixed, g_tune[b][mixed][0], g_tune[b][mixed][1], best_ms);
+    }
+    return rc;
+}
+
+static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
+{
+    RC(init_once());
+    RC(batch_init());
+    tune_load();
+    const int use_graph = env_int("FSX_BATCH_GRAPH", 1, 0, 1);
+    const int tune = env_int("FSX_TUNE", 1, 0, 1);
+    /* the tuner first (it runs in the process-wide buffers) */
+    for (int i = 0; tune && i < nprob; i++) {
+        const int m = (args[i].flags & FSX_MIXED_FP32) != 0, b = tune_bucket(args[i].L);
+        if (!g_tune[b][m][0] && args[i].n > 0) RC(tune_one(&args[i]));
+    }
+    if (g_nbat < nprob) {
+        fsx_bset *nb = (fsx_bset *)realloc(g_bat, sizeof(fsx_bset) * nprob);
+        if (!nb) return FSX_ERR_NOMEM;
+        memset(nb + g_nbat, 0, sizeof(fsx_bset) * (nprob - g_nbat));
+        g_bat = nb;
+        g_nbat = nprob;
+    }
+    fsx_plan *pl = (fsx_plan *)calloc(nprob, sizeof(fsx_plan));
+    int *tail = (int *)calloc(2 * (size_t)nprob, sizeof(int));
+    unsigned long long *tally = (unsigned long long *)calloc(2 * (size_t)nprob, sizeof(unsigned long long));
+    fsx_gkey *key = (fsx_gkey *)calloc(nprob > 0 ? nprob : 1, sizeof(fsx_gkey));
+    int rc = (pl && tail && tally && key) ? FSX_OK : FSX_ERR_NOMEM;
+    int ninit = 0;
+    for (; rc == FSX_OK && ninit < nprob; ninit++) {
+        const fsx_hip_args *A = &args[ninit];
+        const int m = (A->flags & FSX_MIXED_FP32) != 0, b = tune_bucket(A->L);
+        rc = plan_init(&pl[ninit], A, g_tune[b][m][0], g_tune[b][m][1]);
+        if (rc != FSX_OK) break;
+    }
+    /* the shared buffers of each stream at the size its largest complex
+     * needs, before any plan takes their addresses */
+    for (int j = 0; rc == FSX_OK && j < FSX_BSTREAMS; j++)
+        for (int i = 0; rc == FSX_OK && i < P_NBUF; i++) {
+            if (!per_stream_buffer(i)) continue;
+            size_t need = 0;
+            for (int k = j; k < ninit; k += FSX_BSTREAMS)
+                if (pl[k].sz[i] > need) need = pl[k].sz[i];
+            if (need == 0) continue;
+            fsx_bctx bc = {NULL, &g_bstr[j]};
+            void *dummy;
+            rc = get_bat(i, need, &bc, &dummy);
+        }
+    for (int k = 0; rc == FSX_OK && k < ninit; k++) {
+        const fsx_hip_args *A = &args[k];
+        fsx_bctx bc = {&g_bat[k], &g_bstr[k % FSX_BSTREAMS]};
+        rc = plan_bind(&pl[k], get_bat, &bc);
+        if (rc != FSX_OK) break;
+        fsx_bset *bs = &g_bat[k];
+        const int up_dw = !(A->dw_key && bs->dw_key == A->dw_key);
+        const int up_ds = !(A->ds_key && bs->ds_key == A->ds_key);
+        rc = plan_upload(&pl[k], g_stream, up_dw, up_ds);
+        if (rc == FSX_OK) {
+            bs->dw_key = A->dw_key;
+            bs->ds_key = A->ds_key;
+        }
+        fsx_gkey *ky = &key[k];
+        const fsx_plan *p = &pl[k];
+        ky->L = p->L; ky->nb = p->nb; ky->na = p->na; ky->ng = p->ng; ky->Q = p->Q; ky->zn = p->zn;
+        ky->n = p->n; ky->flags = (int)A->flags; ky->half = p->half; ky->npk = p->npk; ky->C = p->C;
+        ky->tile_rows = p->tile_rows; ky->aq_lds = p->aq_lds; ky->lanes = p->lanes;
+        ky->mult = A->mult; ky->peak = A->peak; ky->c1_lo = A->c1_lo; ky->c1_hi = A->c1_hi;
+        ky->c2_lo = A->c2_lo; ky->c2_hi = A->c2_hi;
+        memcpy(ky->d, p->d, sizeof(ky->d));
+        ky->dposes = p->dposes; ky->dscore = p->dscore; ky->dc1 = p->dc1; ky->dc2 = p->dc2;
+        ky->dcross = p->dcross;
+    }
+    /* the stages of every complex: one graph launch, or the launches */
+    if (rc == FSX_OK && nprob > 0) {
+        const size_t klen = sizeof(fsx_gkey) * nprob;
+        const int replay = use_graph && g_gexec && g_gkey_len == klen && memcmp(g_gkey, key, klen) == 0;
+        if (!replay) {
+            graph_drop();
+            hipError_t e = use_graph ? hipStreamBeginCapture(g_stream, hipStreamCaptureModeRelaxed)
+                                     : hipSuccess;
+            if (e == hipSuccess) e = hipEventRecord(g_bfork, g_stream);
+            for (int j = 0; e == hipSuccess && j < FSX_BSTREAMS; j++)
+                e = hipStreamWaitEvent(g_bs[j], g_bfork, 0);
+            if (e != hipSuccess) rc = FSX_ERR_HIP;
+            for (int i = 0; rc == FSX_OK && i < nprob; i++)
+                rc = plan_enqueue(&pl[i], g_bs[i % FSX_BSTREAMS], NULL);
+            for (int j = 0; rc == FSX_OK && j < FSX_BSTREAMS; j++) {
+                if (hipEventRecord(g_bjoin[j], g_bs[j]) != hipSuccess ||
+                    hipStreamWaitEvent(g_stream, g_bjoin[j], 0) != hipSuccess)
+                    rc = FSX_ERR_HIP;
+            }
+            if (use_graph) {
+                hipGraph_t gr = NULL;
+                const hipError_t ee = hipStreamEndCapture(g_stream, &gr);
+                if (rc == FSX_OK && ee != hipSuccess) rc = FSX_ERR_HIP;
+                if (rc == FSX_OK && hipGraphInstantiate(&g_gexec, gr, NULL, NULL, 0) != hipSuccess) {
+                    g_gexec = NULL;
+                    rc = FSX_ERR_HIP;
+                }
+                if (rc == FSX_OK) {
+                    g_graph = gr;
+                    g_gkey = malloc(klen);
+                    if (g_gkey) {
+                        memcpy(g_gkey, key, klen);
+                        g_gkey_len = klen;
+                    }
+                } else if (gr) {
+                    (void)hipGraphDestroy(gr);
+                }
+            }
+        }
+        if (rc == FSX_OK && use_graph && hipGraphLaunch(g_gexec, g_stream) != hipSuccess)
+            rc = FSX_ERR_HIP;
+    }
+    for (int i = 0; rc == FSX_OK && i < nprob; i++)
+        rc = plan_download(&pl[i], g_stream, tail + 2 * i, tally + 2 * i);
+    if (hipStreamSynchronize(g_stream) != hipSuccess && rc == FSX_OK) rc = FSX_ERR_HIP;
+    for (int i = 0; rc == FSX_OK && i < nprob; i++)
+        plan_stats(&pl[i], &st[i], NULL, tail + 2 * i, tally + 2 * i);
+    if (rc != FSX_OK) graph_drop();
+    for (int i = 0; i < ninit; i++) plan_free_host(&pl[i]);
+    free(pl);
+    free(tail);
+    free(tally);
+    free(key);
+    return rc;
+}
+
+extern "C" int fsx_hip_run_batch(int nprob, const fsx_hip_args *args, fsx_stats *st)
+{
+    pthread_mutex_lock(&g_mtx);
+    int rc = run_batch_locked(nprob, args, st);
     pthread_mutex_unlock(&g_mtx);
     return rc;
 }

@@ -68,129 +68,191 @@ static void const_terms(const fsx_problem *pb, double *cint)
         }
 }
 
-/* The Wigner-d tables of the last beta grid, kept across calls (they depend
- * only on L, nbeta and pi); fsx_score calls are serialised on this lock. */
+/* Wigner-d tables of recent beta grids, kept across calls (they depend only
+ * on L, nbeta and pi); fsx_score calls are serialised on this lock. */
+#define FSX_DW_CACHE 8
 static pthread_mutex_t g_dw_lock = PTHREAD_MUTEX_INITIALIZER;
 static struct {
     int L, nbeta;
     double pi;
     double *dW;
-    unsigned long long key;
-} g_dw;
-static unsigned long long g_dw_serial;
+    unsigned long long key, used;
+} g_dw[FSX_DW_CACHE];
+static unsigned long long g_dw_serial, g_dw_clock;
 
-static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
-                            double *c1, double *c2, double *cross, unsigned flags,
-                            fsx_stats *st, double t0);
-
-/* Device buffers and the host tables (Wigner-d of the last grid, d-symbols);
- * the next call builds and uploads them again.  Waits for calls in flight. */
+/* Device buffers and the host tables (Wigner-d, d-symbols); the next call
+ * builds and uploads them again.  Waits for calls in flight. */
 void fsx_release(void)
 {
     pthread_mutex_lock(&g_dw_lock);
-    free(g_dw.dW);
-    g_dw.dW = NULL;
-    g_dw.key = 0;
+    for (int i = 0; i < FSX_DW_CACHE; i++) {
+        free(g_dw[i].dW);
+        g_dw[i].dW = NULL;
+        g_dw[i].key = 0;
+    }
     fsx_d_symb_drop();
     fsx_hip_release();
     pthread_mutex_unlock(&g_dw_lock);
 }
 
+/* the Wigner-d table of (L, nbeta, pi), built on a miss in place of the
+ * least recently used entry; *key identifies its content */
+static const double *dw_table(int L, int nbeta, double pi, unsigned long long *key)
+{
+    int hit = -1, lru = 0;
+    for (int i = 0; i < FSX_DW_CACHE; i++) {
+        if (g_dw[i].dW && g_dw[i].L == L && g_dw[i].nbeta == nbeta &&
+            memcmp(&g_dw[i].pi, &pi, sizeof(double)) == 0)
+            hit = i;
+        if (g_dw[i].used < g_dw[lru].used) lru = i;
+    }
+    if (hit < 0) {
+        const size_t dsz = (size_t)(L + 1) * (2 * L + 1) * (2 * L + 1);
+        double *dW = (double *)malloc(sizeof(double) * dsz * nbeta);
+        if (!dW) return NULL;
+        /* fftsaxs.c:632-636, 706-708: beta_i = 0 + i * (pi - 0)/(nbeta - 1) */
+        const double bstep = (pi - 0.0) / (nbeta - 1);
+        for (int b = 0; b < nbeta; b++) fsx_wigner_d(L, 0.0 + b * bstep, dW + b * dsz);
+        hit = lru;
+        free(g_dw[hit].dW);
+        g_dw[hit].dW = dW;
+        g_dw[hit].L = L;
+        g_dw[hit].nbeta = nbeta;
+        g_dw[hit].pi = pi;
+        g_dw[hit].key = ++g_dw_serial;
+    }
+    g_dw[hit].used = ++g_dw_clock;
+    *key = g_dw[hit].key;
+    return g_dw[hit].dW;
+}
+
+/* per-call host tables of one complex */
+typedef struct {
+    double *cint, *twa, *twg;
+} call_tabs;
+
+static void tabs_free(call_tabs *h)
+{
+    free(h->cint);
+    free(h->twa);
+    free(h->twg);
+    h->cint = h->twa = h->twg = NULL;
+}
+
+/* Checks one complex's arguments and fills the GPU pipeline's arguments
+ * (tables in h, owned by the caller of this function).  Returns FSX_OK,
+ * an error code, or 1 when there is nothing to score. */
+static int prepare(const fsx_problem *pb, const int64_t *poses, int n, double *score, double *c1,
+                   double *c2, double *cross, unsigned flags, fsx_hip_args *a, call_tabs *h)
+{
+    memset(h, 0, sizeof(*h));
+    if (!pb || n < 0 || (n > 0 && (!poses || !score || !c1 || !c2))) return FSX_ERR_ARG;
+    if (pb->L < 1 || pb->L > 40 || pb->nbeta < 2 || pb->nalpha < 1 || pb->ngamma < 1 ||
+        pb->qnum < 2 || pb->znum < 1 || !pb->qvals || !pb->zvals || !pb->A_re ||
+        !pb->A_im || !pb->B_re || !pb->B_im || !pb->a)
+        return FSX_ERR_ARG;
+    if (n == 0) return 1;
+    const int L = pb->L;
+    const double *ds = fsx_d_symb_cached(L);
+    if (!ds) return FSX_ERR_NOMEM;
+    unsigned long long dw_key = 0;
+    const double *dW = dw_table(L, pb->nbeta, pb->pi, &dw_key);
+    h->cint = (double *)malloc(sizeof(double) * 6 * pb->qnum);
+    h->twa = (double *)malloc(sizeof(double) * 2 * pb->nalpha);
+    h->twg = (double *)malloc(sizeof(double) * 2 * pb->ngamma);
+    if (!dW || !h->cint || !h->twa || !h->twg) {
+        tabs_free(h);
+        return FSX_ERR_NOMEM;
+    }
+    const_terms(pb, h->cint);
+    /* exact twiddles e^{-2 pi i k / N} (the FFTW transform of fftsaxs.c:680) */
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int k = 0; k < pb->nalpha; k++) {
+        h->twa[2 * k] = cos(two_pi * k / pb->nalpha);
+        h->twa[2 * k + 1] = -sin(two_pi * k / pb->nalpha);
+    }
+    for (int k = 0; k < pb->ngamma; k++) {
+        h->twg[2 * k] = cos(two_pi * k / pb->ngamma);
+        h->twg[2 * k + 1] = -sin(two_pi * k / pb->ngamma);
+    }
+    memset(a, 0, sizeof(*a));
+    a->L = L; a->nbeta = pb->nbeta; a->nalpha = pb->nalpha; a->ngamma = pb->ngamma;
+    a->qnum = pb->qnum; a->znum = pb->znum; a->qvals = pb->qvals; a->zvals = pb->zvals;
+    a->A_re = pb->A_re; a->A_im = pb->A_im; a->B_re = pb->B_re; a->B_im = pb->B_im;
+    a->a = pb->a; a->cint = h->cint; a->dW = dW; a->dsymb = ds; a->tw_a = h->twa; a->tw_g = h->twg;
+    a->dw_key = dw_key;
+    a->ds_key = (unsigned long long)L + 1;   /* the d-symbol table of L never changes */
+    a->mult = pb->mult; a->peak = pb->peak;
+    a->c1_lo = pb->c1_lo; a->c1_hi = pb->c1_hi; a->c2_lo = pb->c2_lo; a->c2_hi = pb->c2_hi;
+    a->poses = poses; a->n = n; a->score = score; a->c1 = c1; a->c2 = c2; a->cross = cross;
+    a->flags = flags;
+    return FSX_OK;
+}
+
 int fsx_score(const fsx_problem *pb, const int64_t *poses, int n, double *score,
               double *c1, double *c2, double *cross, unsigned flags, fsx_stats *st)
 {
-    double t0 = now_ms();
+    const double t0 = now_ms();
+    fsx_stats local;
+    if (!st) st = &local;
+    memset(st, 0, sizeof(*st));
     pthread_mutex_lock(&g_dw_lock);
-    int rc = fsx_score_locked(pb, poses, n, score, c1, c2, cross, flags, st, t0);
+    fsx_hip_args a;
+    call_tabs h;
+    int rc = prepare(pb, poses, n, score, c1, c2, cross, flags, &a, &h);
+    if (rc == FSX_OK) {
+        st->ms_tables = now_ms() - t0;
+        rc = fsx_hip_run(&a, st);
+    } else if (rc == 1) {
+        rc = FSX_OK;
+    }
+    tabs_free(&h);
     pthread_mutex_unlock(&g_dw_lock);
+    st->ms_total = now_ms() - t0;
     return rc;
 }
 
+/* The complexes are prepared on the host, then scored by one batched GPU
+ * call (fsx_hip_run_batch): per-complex stage times are not measured
+ * there, and every record's ms_total is the whole call's. */
 int fsx_score_batch(int nprob, const fsx_problem *pbs, const int64_t *const *poses,
                     const int *n, double *const *score, double *const *c1, double *const *c2,
                     double *const *cross, unsigned flags, fsx_stats *stats)
 {
     if (nprob < 0 || (nprob > 0 && (!pbs || !poses || !n || !score || !c1 || !c2)))
         return FSX_ERR_ARG;
-    pthread_mutex_lock(&g_dw_lock);
-    int rc = FSX_OK;
-    for (int i = 0; i < nprob && rc == FSX_OK; i++)
-        rc = fsx_score_locked(&pbs[i], poses[i], n[i], score[i], c1[i], c2[i],
-                              cross ? cross[i] : NULL, flags, stats ? &stats[i] : NULL,
-                              now_ms());
-    pthread_mutex_unlock(&g_dw_lock);
-    return rc;
-}
-
-static int fsx_score_locked(const fsx_problem *pb, const int64_t *poses, int n, double *score,
-                            double *c1, double *c2, double *cross, unsigned flags,
-                            fsx_stats *st, double t0)
-{
-    fsx_stats local;
-    if (!st) st = &local;
-    memset(st, 0, sizeof(*st));
-    if (!pb || n < 0 || (n > 0 && (!poses || !score || !c1 || !c2))) return FSX_ERR_ARG;
-    if (pb->L < 1 || pb->L > 40 || pb->nbeta < 2 || pb->nalpha < 1 || pb->ngamma < 1 ||
-        pb->qnum < 2 || pb->znum < 1 || !pb->qvals || !pb->zvals || !pb->A_re ||
-        !pb->A_im || !pb->B_re || !pb->B_im || !pb->a)
-        return FSX_ERR_ARG;
-    if (n == 0) return FSX_OK;
-    int L = pb->L, K = L + 1, N = 2 * L + 1;
-    size_t dsz = (size_t)K * N * N;
-    const double *ds = fsx_d_symb_cached(L);
-    if (!ds) return FSX_ERR_NOMEM;
-    double *cint = (double *)malloc(sizeof(double) * 6 * pb->qnum);
-    double *twa = (double *)malloc(sizeof(double) * 2 * pb->nalpha);
-    double *twg = (double *)malloc(sizeof(double) * 2 * pb->ngamma);
-    if (!cint || !twa || !twg) {
-        free(cint); free(twa); free(twg);
+    const double t0 = now_ms();
+    if (stats) memset(stats, 0, sizeof(fsx_stats) * (size_t)nprob);
+    if (nprob == 0) return FSX_OK;
+    fsx_hip_args *a = (fsx_hip_args *)calloc(nprob, sizeof(fsx_hip_args));
+    call_tabs *h = (call_tabs *)calloc(nprob, sizeof(call_tabs));
+    int *idx = (int *)calloc(nprob, sizeof(int));   /* complex of each batch entry */
+    fsx_stats *st = (fsx_stats *)calloc(nprob, sizeof(fsx_stats));
+    if (!a || !h || !idx || !st) {
+        free(a); free(h); free(idx); free(st);
         return FSX_ERR_NOMEM;
     }
-    const_terms(pb, cint);
-    if (!(g_dw.dW && g_dw.L == L && g_dw.nbeta == pb->nbeta &&
-          memcmp(&g_dw.pi, &pb->pi, sizeof(double)) == 0)) {
-        double *dW = (double *)malloc(sizeof(double) * dsz * pb->nbeta);
-        if (!dW) {
-            free(cint); free(twa); free(twg);
-            return FSX_ERR_NOMEM;
+    pthread_mutex_lock(&g_dw_lock);
+    int rc = FSX_OK, nb = 0;
+    for (int i = 0; i < nprob && rc == FSX_OK; i++) {
+        const int r = prepare(&pbs[i], poses[i], n[i], score[i], c1[i], c2[i],
+                              cross ? cross[i] : NULL, flags, &a[nb], &h[nb]);
+        if (r == FSX_OK) idx[nb++] = i;
+        else if (r != 1) rc = r;
+    }
+    const double t1 = now_ms();
+    if (rc == FSX_OK && nb > 0) rc = fsx_hip_run_batch(nb, a, st);
+    for (int k = 0; k < nprob; k++) tabs_free(&h[k]);
+    pthread_mutex_unlock(&g_dw_lock);
+    const double t2 = now_ms();
+    if (stats && rc == FSX_OK)
+        for (int k = 0; k < nb; k++) {
+            stats[idx[k]] = st[k];
+            stats[idx[k]].ms_tables = t1 - t0;
         }
-        /* fftsaxs.c:632-636, 706-708: beta_i = 0 + i * (pi - 0)/(nbeta - 1) */
-        double bstep = (pb->pi - 0.0) / (pb->nbeta - 1);
-        for (int b = 0; b < pb->nbeta; b++) fsx_wigner_d(L, 0.0 + b * bstep, dW + b * dsz);
-        free(g_dw.dW);
-        g_dw.dW = dW;
-        g_dw.L = L;
-        g_dw.nbeta = pb->nbeta;
-        g_dw.pi = pb->pi;
-        g_dw.key = ++g_dw_serial;
-    }
-    const double *dW = g_dw.dW;
-    /* exact twiddles e^{-2 pi i k / N} (the FFTW transform of fftsaxs.c:680) */
-    const double two_pi = 6.283185307179586476925286766559;
-    for (int k = 0; k < pb->nalpha; k++) {
-        twa[2 * k] = cos(two_pi * k / pb->nalpha);
-        twa[2 * k + 1] = -sin(two_pi * k / pb->nalpha);
-    }
-    for (int k = 0; k < pb->ngamma; k++) {
-        twg[2 * k] = cos(two_pi * k / pb->ngamma);
-        twg[2 * k + 1] = -sin(two_pi * k / pb->ngamma);
-    }
-    fsx_hip_args a;
-    memset(&a, 0, sizeof(a));
-    a.L = L; a.nbeta = pb->nbeta; a.nalpha = pb->nalpha; a.ngamma = pb->ngamma;
-    a.qnum = pb->qnum; a.znum = pb->znum; a.qvals = pb->qvals; a.zvals = pb->zvals;
-    a.A_re = pb->A_re; a.A_im = pb->A_im; a.B_re = pb->B_re; a.B_im = pb->B_im;
-    a.a = pb->a; a.cint = cint; a.dW = dW; a.dsymb = ds; a.tw_a = twa; a.tw_g = twg;
-    a.dw_key = g_dw.key;
-    a.ds_key = (unsigned long long)L + 1;   /* the d-symbol table of L never changes */
-    a.mult = pb->mult; a.peak = pb->peak;
-    a.c1_lo = pb->c1_lo; a.c1_hi = pb->c1_hi; a.c2_lo = pb->c2_lo; a.c2_hi = pb->c2_hi;
-    a.poses = poses; a.n = n; a.score = score; a.c1 = c1; a.c2 = c2; a.cross = cross;
-    a.flags = flags;
-    st->ms_tables = now_ms() - t0;
-    int rc = fsx_hip_run(&a, st);
-    free(cint); free(twa); free(twg);
-    st->ms_total = now_ms() - t0;
+    if (stats)
+        for (int i = 0; i < nprob; i++) stats[i].ms_total = t2 - t0;
+    free(a); free(h); free(idx); free(st);
     return rc;
 }
 

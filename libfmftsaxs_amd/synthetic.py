@@ -82,46 +82,61 @@ def shrake_rupley(mol, probe=1.4, npts=96, seed=7, occluders="all"):
     """Fraction of solvent-accessible surface per atom (Shrake-Rupley, probe
     1.4 A, Fibonacci sphere of npts points on r_vdw + probe), in the
     normalisation sxs_faccs returns (saxs_utils.c:30-48): contact area over
-    4 pi r_vdw^2, i.e. the unburied fraction of the probe sphere."""
-    rng = np.random.default_rng(seed)
+    4 pi r_vdw^2, i.e. the unburied fraction of the probe sphere.  The
+    neighbours of an atom are the atoms within R_i + R_j of it (no other
+    atom can bury a point of its sphere); atoms are processed in chunks with
+    their neighbour lists padded, the test per (point, neighbour) being the
+    squared distance against R_j^2."""
+    del seed
     # Fibonacci sphere
     k = np.arange(npts) + 0.5
     phi = np.arccos(1 - 2 * k / npts)
     th = np.pi * (1 + 5 ** 0.5) * k
     sph = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
-    del rng
     X = mol.coords
     R = mol.vdw + probe
     n = X.shape[0]
-    sa = np.zeros(n)
-    cell = 2 * R.max()
-    keys = np.floor(X / cell).astype(np.int64)
-    from collections import defaultdict
-    grid = defaultdict(list)
-    for i, kk in enumerate(map(tuple, keys)):
-        grid[kk].append(i)
-    for i in range(n):
-        kx, ky, kz = keys[i]
-        nb = []
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    nb.extend(grid.get((kx + dx, ky + dy, kz + dz), []))
-        nb = np.array([j for j in nb if j != i and (occluders == "all" or mol.vdw[j] > 0)],
-                      dtype=np.int64)
-        pts = X[i] + R[i] * sph
-        if nb.size:
-            d2 = ((pts[:, None, :] - X[nb][None, :, :]) ** 2).sum(-1)
-            buried = (d2 < (R[nb] ** 2)[None, :]).any(1)
-            free = npts - buried.sum()
-        else:
-            free = npts
-        ri = mol.vdw[i]
-        # contact surface (the accessible fraction scaled to the vdW sphere,
-        # sxs_faccs calls accs with cont_acc = 1), divided by 4 pi r^2 as in
-        # saxs_utils.c:40-47; radius-0 atoms (hydrogens) carry no SASA
-        sa[i] = 0.0 if ri == 0 else free / npts
-    return sa
+    occl = np.ones(n, dtype=bool) if occluders == "all" else mol.vdw > 0
+    # neighbours: atoms closer than R_i + R_j (only they can bury a point of
+    # atom i's sphere; a generous margin keeps every such atom)
+    from scipy.spatial import cKDTree
+    pairs = cKDTree(X).query_pairs(2 * R.max() + 1e-6, output_type="ndarray")
+    pi, pj = pairs[:, 0], pairs[:, 1]
+    d = np.sqrt(((X[pi] - X[pj]) ** 2).sum(1))
+    keep = d < R[pi] + R[pj] + 1e-6
+    src = np.concatenate([pi[keep], pj[keep]])
+    dst = np.concatenate([pj[keep], pi[keep]])
+    sel = occl[dst]
+    src, dst = src[sel], dst[sel]
+    order = np.argsort(src, kind="stable")
+    src, dst = src[order], dst[order]
+    starts = np.searchsorted(src, np.arange(n + 1))
+    nbl = [dst[starts[i]:starts[i + 1]] for i in range(n)]
+    free = np.full(n, npts, dtype=np.int64)
+    chunk = 64
+    for c0 in range(0, n, chunk):
+        idx = np.arange(c0, min(n, c0 + chunk))
+        mx = max(1, max(nbl[i].size for i in idx))
+        nb = np.zeros((idx.size, mx), dtype=np.int64)
+        valid = np.zeros((idx.size, mx), dtype=bool)
+        for r, i in enumerate(idx):
+            nb[r, :nbl[i].size] = nbl[i]
+            valid[r, :nbl[i].size] = True
+        pts = X[idx][:, None, :] + R[idx][:, None, None] * sph[None, :, :]   # (c, npts, 3)
+        xn = X[nb]                                                            # (c, mx, 3)
+        # squared distances (c, npts, mx), summed over x, y, z in that order
+        dx = pts[:, :, None, 0] - xn[:, None, :, 0]
+        d2 = dx * dx
+        dx = pts[:, :, None, 1] - xn[:, None, :, 1]
+        d2 += dx * dx
+        dx = pts[:, :, None, 2] - xn[:, None, :, 2]
+        d2 += dx * dx
+        buried = ((d2 < (R[nb] ** 2)[:, None, :]) & valid[:, None, :]).any(2)
+        free[idx] = npts - buried.sum(1)
+    # contact surface (the accessible fraction scaled to the vdW sphere,
+    # sxs_faccs calls accs with cont_acc = 1), divided by 4 pi r^2 as in
+    # saxs_utils.c:40-47; radius-0 atoms (hydrogens) carry no SASA
+    return np.where(mol.vdw == 0, 0.0, free / npts)
 
 
 def make_dimer_inputs(n_rec, n_lig, seed_rec, seed_lig, water=True):
