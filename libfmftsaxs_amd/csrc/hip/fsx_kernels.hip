@@ -40,6 +40,7 @@
 
 struct cplx { double re, im; };
 struct cplx32 { float re, im; };   /* element of the mixed-precision (FP32) tables */
+typedef double fsx_v4d __attribute__((ext_vector_type(4)));   /* FP64 MFMA 16x16x4 result */
 
 /* ---------------------------------------------------------------------- */
 /* device helpers */
@@ -425,6 +426,95 @@ k_tab_transpose(E *out, const E *tmp, int cs, int nrows, int nte, int ntr, int n
             else if (e & 1) d[e >> 1].im = v.re;
             else d[e >> 1].re = v.re;
         }
+    }
+}
+
+/* Rotated table on FP64 MFMA, written in the cross-term kernel's layout in
+ * one pass (no staging table, no transpose).  One block of four waves per
+ * (q, X, beta, tile of GT gamma): for every l, the (m x m1) block of
+ * d^l(beta) times the (m1 x gamma, re/im) block of twiddled source rows
+ * src[q][X][l m1][gamma] (16x16x4 FP64 MFMA: lane i feeds A[i % 16][i / 16],
+ * B[i / 16][i % 16]; result register j of lane i is D[i / 16 + 4j][i % 16]),
+ * rows m = max(mlo, -l)..l, each block's m rows and 16 columns (8 gamma, real
+ * and imaginary parts) a job of one wave.  The results go to an LDS tile of
+ * the block's GT rows of the table, all entries; the tile is written out row
+ * by row, turned by i^(lsgn l) and with the npk packed m = 0 slot pairs of
+ * chunk 0, exactly as k_tab_transpose writes them. */
+template <typename E, int GT>
+__global__ void __launch_bounds__(256)
+k_rotate_mfma(E *out, const cplx *srct, const double *dW, const int *ent, const long *coff,
+              int nbeta, int ng, int Q, int L, int mlo, int ne, int npk, int lsgn)
+{
+    extern __shared__ double smem_raw[];
+    E *tile = reinterpret_cast<E *>(smem_raw);   /* [GT][ne] */
+    const int K = L + 1, K2 = K * K, N = 2 * L + 1;
+    const int *ent_l = ent, *ent_c = ent + 2 * K2, *ent_ec = ent + 3 * K2, *csize = ent + 4 * K2;
+    const int *mstart = ent + 5 * K2;
+    const int nrows = nbeta * ng, ngt = (ng + GT - 1) / GT;
+    int r = blockIdx.x;
+    const int gt = r % ngt; r /= ngt;
+    const int b = r % nbeta; r /= nbeta;
+    const int X = r % 3;
+    const int q = r / 3;
+    const int g0 = gt * GT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NT = GT / 8;   /* 16-column tiles: 8 gamma x (re, im) each */
+    const double *dWb = dW + (size_t)b * K * N * N;
+    const cplx *srcq = srct + (size_t)(q * 3 + X) * K2 * ng;
+    /* jobs (l, m-tile, column tile), dealt to the waves in turn */
+    int j = 0;
+    for (int l = 0; l <= L; l++) {
+        const int mb = mlo > -l ? mlo : -l, nm = l - mb + 1, nmt = (nm + 15) / 16;
+        const int njob = nmt * NT;
+        for (int jj = 0; jj < njob; jj++, j++) {
+            if ((j & 3) != w) continue;
+            const int mt = jj / NT, nt = jj - mt * NT;
+            const int i = lane & 15, k = lane >> 4;
+            const int m = mb + mt * 16 + i;
+            const int gl = nt * 8 + (i >> 1), part = i & 1, g = g0 + gl;
+            const bool arow = m <= l, bcol = gl < GT && g < ng;
+            const double *drow = dWb + ((size_t)l * N + (L + (arow ? m : l))) * N + (L - l);
+            const double *scol = reinterpret_cast<const double *>(srcq + (size_t)l * l * ng +
+                                                                  (bcol ? g : g0)) + part;
+            fsx_v4d acc = {0.0, 0.0, 0.0, 0.0};
+            for (int kk = 0; kk < 2 * l + 1; kk += 4) {
+                const int m1i = kk + k;
+                const bool kin = m1i <= 2 * l;
+                const double av = arow && kin ? drow[m1i] : 0.0;
+                const double bv = bcol && kin ? scol[(size_t)m1i * ng * 2] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+            const int c = lane & 15;
+            const int gl2 = nt * 8 + (c >> 1), part2 = c & 1;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int m2 = mb + mt * 16 + (lane >> 4) + 4 * rr;
+                if (m2 <= l && gl2 < GT) {
+                    const int e = mstart[m2 + L] + (l - (m2 < 0 ? -m2 : m2));
+                    decltype(E::re) *t = reinterpret_cast<decltype(E::re) *>(&tile[gl2 * ne + e]);
+                    t[part2] = (decltype(E::re))acc[rr];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < GT * ne; idx += blockDim.x) {
+        const int gl = idx / ne, e = idx - gl * ne, g = g0 + gl;
+        if (g >= ng) continue;
+        E v = tile[idx];
+        if (lsgn != 0) {   /* times i^(lsgn l), l of the entry */
+            const int ph = (lsgn > 0 ? ent_l[e] : -ent_l[e]) & 3;
+            const decltype(v.re) x = v.re, y = v.im;
+            v.re = ph == 0 ? x : ph == 1 ? -y : ph == 2 ? -x : y;
+            v.im = ph == 0 ? y : ph == 1 ? x : ph == 2 ? -y : -x;
+        }
+        const int c = ent_c[e], ec = ent_ec[e], cs = csize[c];
+        E *d = out + coff[c] + ((size_t)((size_t)q * nrows + b * ng + g) * 3 + X) * cs;
+        const int pk = c == 0 ? npk : 0;
+        /* packed m = 0 pairs: i^l R_l0 is real for every l */
+        if (ec >= 2 * pk) d[ec - pk] = v;
+        else if (ec & 1) d[ec >> 1].im = v.re;
+        else d[ec >> 1].re = v.re;
     }
 }
 
@@ -1065,7 +1155,6 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
  * then k_atom_finish sums the chunks in order and applies 4 pi i^l. */
 #define FSX_AT_CHUNK 1024   /* atoms per k_atom_mfma block (4 waves x 256) */
 #define FSX_AT_MAXT 16      /* column tiles: 3 * 2 (L + 1) <= 16 * 16 for L <= 41 */
-typedef double fsx_v4d __attribute__((ext_vector_type(4)));
 
 __global__ void k_atom_geom(const double *xyz, const double *vac, const double *dum,
                             const double *sa, double h2o_ff, int na, int L, double pi_ref,
@@ -1573,6 +1662,8 @@ struct fsx_plan {
     long lanes;
     fit_kernel_t kfit;
     int tile_rows;       /* gamma2 rows per cross-term tile (0: the form's default) */
+    int rot_gt;          /* gamma per block of the MFMA rotation (16 or 8), 0: the
+                            rotation-then-transpose kernels */
 };
 
 static void plan_free_host(fsx_plan *p)
@@ -1692,6 +1783,14 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     z[P_BES] = sizeof(double) * zn * Q * N;
     z[P_T] = es * zn * Q * (size_t)p->Tn;
     z[P_RT] = z[P_BT] = z[P_RTMP] = es * (size_t)p->nrows * spfe;
+    /* the MFMA rotation when a tile of 16 (else 8) rows of the table fits
+     * 64 KB of LDS (FSX_ROT_MFMA=0: the rotation-then-transpose kernels) */
+    p->rot_gt = 0;
+    if (env_int("FSX_ROT_MFMA", 1, 0, 1)) {
+        if ((size_t)16 * ne * es <= 64 * 1024) p->rot_gt = 16;
+        else if ((size_t)8 * ne * es <= 64 * 1024) p->rot_gt = 8;
+    }
+    if (p->rot_gt) z[P_RTMP] = 0;
     z[P_A] = z[P_B] = sizeof(double) * 2 * p->spf;
     z[P_DW] = sizeof(double) * p->dsz * nb;
     z[P_DS] = sizeof(double) * (size_t)K * K2 * N;
@@ -1849,6 +1948,31 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
             hipLaunchKernelGGL(k_twiddle, dim3(nblk(ntw, 256)), dim3(256), 0, s, srct, src,
                                src + spf, (const double *)g[P_TWG], ng, Q, L, side);
             const int npk = side == 0 ? p->npk : 0, lrot = side == 0 ? 1 : -1;
+            if (p->rot_gt) {
+                const int gt = p->rot_gt;
+                const long nbr = (long)Q * 3 * nb * ((ng + gt - 1) / gt);
+                const size_t lds = (size_t)gt * p->ne * p->es;
+                const double *dw = (const double *)g[P_DW];
+                const int *en = (const int *)g[P_ENT];
+                const long *co = (const long *)g[P_COFF];
+                if (p->mixed && gt == 16)
+                    hipLaunchKernelGGL((k_rotate_mfma<cplx32, 16>), dim3((unsigned)nbr), dim3(256), lds, s,
+                                       (cplx32 *)out, (const cplx *)srct, dw, en, co, nb, ng, Q, L,
+                                       p->mlo, p->ne, npk, lrot);
+                else if (p->mixed)
+                    hipLaunchKernelGGL((k_rotate_mfma<cplx32, 8>), dim3((unsigned)nbr), dim3(256), lds, s,
+                                       (cplx32 *)out, (const cplx *)srct, dw, en, co, nb, ng, Q, L,
+                                       p->mlo, p->ne, npk, lrot);
+                else if (gt == 16)
+                    hipLaunchKernelGGL((k_rotate_mfma<cplx, 16>), dim3((unsigned)nbr), dim3(256), lds, s,
+                                       (cplx *)out, (const cplx *)srct, dw, en, co, nb, ng, Q, L,
+                                       p->mlo, p->ne, npk, lrot);
+                else
+                    hipLaunchKernelGGL((k_rotate_mfma<cplx, 8>), dim3((unsigned)nbr), dim3(256), lds, s,
+                                       (cplx *)out, (const cplx *)srct, dw, en, co, nb, ng, Q, L,
+                                       p->mlo, p->ne, npk, lrot);
+                continue;
+            }
             if (p->mixed)
                 launch_rotate<cplx32>(s, out, g[P_RTMP], srct, nrot, lds_rot, nb, ng, Q, L, gc, C,
                                       csz, cst, coff, p->mlo, npk, lrot, g);
