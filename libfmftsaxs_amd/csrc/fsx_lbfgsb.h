@@ -154,8 +154,15 @@ FSX_HD void fsx_ct_get6(const fsx_ct *t, int i, double *o)
 }
 
 /* Terms at q = i0 .. i0+nb-1 (nb <= FSX_QB) into blk: all raw loads first,
- * then (c + v) * scale per value exactly as fsx_ct_get. */
-#define FSX_QB 4
+ * then (c + v) * scale per value exactly as fsx_ct_get.  One q per block:
+ * the GPU fit kernel runs the evaluation and the whole L-BFGS-B update in
+ * one loop, and every copy of a q step the unrolled loops make adds to code
+ * that the instruction cache (64 KB per pair of CUs) must hold; with four q
+ * per block the kernel's loop no longer fitted and the fit ran 5.5 ms
+ * instead of 5.2 ms on the 210,000-pose workload. */
+#ifndef FSX_QB
+#define FSX_QB 1
+#endif
 FSX_HD void fsx_ct_block(const fsx_ct *t, int i0, int nb, double blk[FSX_QB][6])
 {
     if (!t->c) {
@@ -259,6 +266,58 @@ FSX_HD double fsx_in(const fsx_ct *t, int i, double G, double c2)
 #define FSX_QLOOP_UNROLL
 #endif
 
+/* The per-q terms of the two passes, written once so that every evaluation
+ * (the sequential q loops below and the GPU's wavefront-wide evaluation,
+ * which forms the terms of many q at once and sums them in q order) forms
+ * them by the same operations in the same order.
+ *
+ * FSX_BS_TERMS: what one q step of the scale pass adds to up and down
+ * (min_saxs.c:302-310), from I at this q and the previous one. */
+#define FSX_BS_TERMS(in, in_prev, qc, q_prev, ai, ut, dt)                             \
+    do {                                                                              \
+        const double tan_ = ((in) - (in_prev)) / ((qc) - (q_prev));                   \
+        const double buf_ = (in) - tan_ * (qc);                                       \
+        (ut) = buf_ * (ai)[1] + tan_ * (ai)[2];                                       \
+        (dt) = buf_ * buf_ * (ai)[3] + 2.0 * tan_ * buf_ * (ai)[4]                    \
+             + tan_ * tan_ * (ai)[5];                                                 \
+    } while (0)
+
+/* FSX_OBJ_VALS: I and its derivatives in c1 and c2 at one q (min_saxs.c:
+ * 55-68), from the six terms cur, G, Gd = dG/dc1 and c2. */
+#define FSX_OBJ_VALS(cur, G, Gd, c2, in, d1, d2)                                      \
+    do {                                                                              \
+        const double vd_ = (cur)[1], vw_ = (cur)[2], dd_ = (cur)[3];                  \
+        const double dw_ = (cur)[4], ww_ = (cur)[5];                                  \
+        (in) = (cur)[0] - (G) * vd_ + (c2) * vw_ + (G) * (G) * dd_ - (G) * (c2) * dw_ \
+             + (c2) * (c2) * ww_;                                                     \
+        (d1) = (Gd) * (-vd_ + 2.0 * (G) * dd_ - (c2) * dw_);                          \
+        (d2) = vw_ - (G) * dw_ + 2.0 * (c2) * ww_;                                    \
+    } while (0)
+
+/* FSX_OBJ_TERMS: what one q step of the objective adds to g0, g1 and the
+ * score (min_saxs.c:70-93), from the values at this q and the previous
+ * one, rq = 1 / (q_i - q_{i-1}) and the scale k. */
+#define FSX_OBJ_TERMS(in, d1, d2, in_prev, d1p, d2p, rq, qc, ai, k, tg0, tg1, tsc)     \
+    do {                                                                              \
+        const double tan_ = ((in) - (in_prev)) * (rq);                                \
+        const double t1_ = ((d1) - (d1p)) * (rq);                                     \
+        const double t2_ = ((d2) - (d2p)) * (rq);                                     \
+        (tg0) = 2.0 * (k) * (-((d1) - t1_ * (qc)) * (ai)[1] - t1_ * (ai)[2]           \
+                             + (k) * (((in) - tan_ * (qc)) * ((d1) - t1_ * (qc)) * (ai)[3] \
+                                      + ((in) * t1_ + (d1) * tan_ - 2.0 * tan_ * t1_ * (qc)) \
+                                            * (ai)[4]                                 \
+                                      + tan_ * t1_ * (ai)[5]));                       \
+        (tg1) = 2.0 * (k) * (-((d2) - t2_ * (qc)) * (ai)[1] - t2_ * (ai)[2]           \
+                             + (k) * (((in) - tan_ * (qc)) * ((d2) - t2_ * (qc)) * (ai)[3] \
+                                      + ((in) * t2_ + (d2) * tan_ - 2.0 * tan_ * t2_ * (qc)) \
+                                            * (ai)[4]                                 \
+                                      + tan_ * t2_ * (ai)[5]));                       \
+        const double b_ = (in) - tan_ * (qc);                                         \
+        (tsc) = (ai)[0] + (k) * (-2.0 * b_ * (ai)[1] - 2.0 * tan_ * (ai)[2]           \
+                                 + (k) * (b_ * b_ * (ai)[3] + 2.0 * b_ * tan_ * (ai)[4] \
+                                          + tan_ * tan_ * (ai)[5]));                  \
+    } while (0)
+
 /* min_saxs.c:261-319.  bA holds the first block of the profile (q = 0 ..
  * FSX_QB-1) on entry; it is overwritten. */
 FSX_HD double fsx_best_scale_blk(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2,
@@ -276,11 +335,10 @@ FSX_HD double fsx_best_scale_blk(const fsx_ct *t, const fsx_fitpar *p, double c1
         double qc = q[i];                                                             \
         G = c1c * exp(corr * qc * qc);                                                \
         double in = FSX_IN(cur, G, c2);                                               \
-        double tan_ = (in - in_prev) / (qc - q_prev);                                 \
-        double buf = in - tan_ * qc;                                                  \
-        up += buf * a[(i) * 6 + 1] + tan_ * a[(i) * 6 + 2];                           \
-        down += buf * buf * a[(i) * 6 + 3] + 2.0 * tan_ * buf * a[(i) * 6 + 4]        \
-              + tan_ * tan_ * a[(i) * 6 + 5];                                         \
+        double ut, dt;                                                                \
+        FSX_BS_TERMS(in, in_prev, qc, q_prev, a + (i) * 6, ut, dt);                   \
+        up += ut;                                                                     \
+        down += dt;                                                                   \
         in_prev = in;                                                                 \
         q_prev = qc;                                                                  \
     } while (0)
@@ -308,8 +366,10 @@ FSX_HD void fsx_objective_blk(const fsx_ct *t, const fsx_fitpar *p, double k, co
     double c1 = x[0], c2 = x[1];
     double g0 = 0.0, g1 = 0.0, mult = p->mult;
     double corr = -mult * (c1 * c1 - 1.0);
+    /* 3 / c1 and 2 c1 mult of dG/dc1, the same at every q */
+    const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
     double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
-    double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * q[0] * q[0]);
+    double Gd = G * (r3c1 - c1m2 * q[0] * q[0]);
     double in_prev = FSX_IN(v0, G, c2);
     double d1p = -Gd * v0[1] + 2.0 * G * Gd * v0[3] - Gd * c2 * v0[4];
     double d2p = v0[2] - G * v0[4] + 2.0 * c2 * v0[5];
@@ -318,29 +378,14 @@ FSX_HD void fsx_objective_blk(const fsx_ct *t, const fsx_fitpar *p, double k, co
     do {                                                                              \
         double qc = q[i];                                                             \
         G = c1c * exp(corr * qc * qc);                                                \
-        Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);                              \
-        double vd = (cur)[1], vw = (cur)[2], dd = (cur)[3];                           \
-        double dw = (cur)[4], ww = (cur)[5];                                          \
-        double in = (cur)[0] - G * vd + c2 * vw + G * G * dd - G * c2 * dw + c2 * c2 * ww; \
-        double d1 = Gd * (-vd + 2.0 * G * dd - c2 * dw);                              \
-        double d2 = vw - G * dw + 2.0 * c2 * ww;                                      \
-        double buf = rdq ? rdq[i] : 1.0 / (qc - q_prev);                              \
-        double tan_ = (in - in_prev) * buf;                                           \
-        double t1 = (d1 - d1p) * buf;                                                 \
-        double t2 = (d2 - d2p) * buf;                                                 \
-        const double *ai = a + (i) * 6;                                               \
-        g0 += 2.0 * k * (-(d1 - t1 * qc) * ai[1] - t1 * ai[2]                         \
-                         + k * ((in - tan_ * qc) * (d1 - t1 * qc) * ai[3]             \
-                                + (in * t1 + d1 * tan_ - 2.0 * tan_ * t1 * qc) * ai[4] \
-                                + tan_ * t1 * ai[5]));                                \
-        g1 += 2.0 * k * (-(d2 - t2 * qc) * ai[1] - t2 * ai[2]                         \
-                         + k * ((in - tan_ * qc) * (d2 - t2 * qc) * ai[3]             \
-                                + (in * t2 + d2 * tan_ - 2.0 * tan_ * t2 * qc) * ai[4] \
-                                + tan_ * t2 * ai[5]));                                \
-        buf = in - tan_ * qc;                                                         \
-        score += ai[0] + k * (-2.0 * buf * ai[1] - 2.0 * tan_ * ai[2]                 \
-                              + k * (buf * buf * ai[3] + 2.0 * buf * tan_ * ai[4]     \
-                                     + tan_ * tan_ * ai[5]));                         \
+        Gd = G * (r3c1 - c1m2 * qc * qc);                                             \
+        double in, d1, d2, tg0, tg1, tsc;                                             \
+        FSX_OBJ_VALS(cur, G, Gd, c2, in, d1, d2);                                     \
+        const double rq = rdq ? rdq[i] : 1.0 / (qc - q_prev);                         \
+        FSX_OBJ_TERMS(in, d1, d2, in_prev, d1p, d2p, rq, qc, a + (i) * 6, k, tg0, tg1, tsc); \
+        g0 += tg0;                                                                    \
+        g1 += tg1;                                                                    \
+        score += tsc;                                                                 \
         in_prev = in;                                                                 \
         d1p = d1;                                                                     \
         d2p = d2;                                                                     \

@@ -1039,30 +1039,185 @@ __device__ __forceinline__ void fit_begin(fsx_ct *t, fsx_lbrc *S, const fsx_fitp
     fsx_lb_start(S, fp, 1.0, 0.0);
 }
 
-/* LDS of one fit block (256 lanes): the L-BFGS-B K blocks, then (when they
- * fit, aq_lds) the moments, the q grid and the reciprocals 1/(q_i - q_{i-1}) */
+/* LDS of one fit block (256 lanes): the L-BFGS-B K blocks, the sums of the
+ * wavefront-wide evaluation (FSX_WIDE_SUMS x 64 per wavefront), then (when
+ * they fit, aq_lds) the moments, the q grid and the reciprocals
+ * 1/(q_i - q_{i-1}) */
 #define FSX_FIT_LDS (160 * 1024)
-static inline size_t fit_lds_k(void) { return (size_t)FSX_LB_KN * FSX_LB_KSTRIDE * sizeof(double); }
+#define FSX_WIDE_SUMS 3
+static inline size_t fit_lds_k(void)
+{
+    return (size_t)(FSX_LB_KN * FSX_LB_KSTRIDE + 4 * FSX_WIDE_SUMS * 64) * sizeof(double);
+}
 static inline int fit_aq_lds(int Q) { return fit_lds_k() + 8 * (size_t)Q * sizeof(double) <= FSX_FIT_LDS; }
 static inline size_t fit_lds(int Q) { return fit_lds_k() + (fit_aq_lds(Q) ? 8 * (size_t)Q * sizeof(double) : 0); }
+
+/* lane l's value of v, l the same in every lane */
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+/* LDS written by some lanes of this wavefront, read by others */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* sums[c] += row c of sW (nq values, in order) in lane c < nc: the q-ordered
+ * sums of the wide evaluation */
+__device__ __forceinline__ void wide_sum(const double *sW, int nq, int nc, double *sum)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane < nc) {
+        const double *row = sW + lane * 64;
+        double c = *sum;
+        int i = 0;
+        for (; i + 8 <= nq; i += 8) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = row[i + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) c = c + t[u];
+        }
+        for (; i < nq; i++) c = c + row[i];
+        *sum = c;
+    }
+}
+
+/* Evaluation of the fits of the lanes in amask (all of this wavefront) by
+ * the whole wavefront, one fit at a time: lane i holds q = 64 r + i in
+ * round r, forms that q's terms (FSX_BS_TERMS, FSX_OBJ_VALS, FSX_OBJ_TERMS,
+ * with the previous q's values from its neighbour lane) and writes them to
+ * sW; lane c then adds row c to its running sum in q order.  Every value is
+ * therefore formed by the operations of the sequential loop and every sum
+ * adds the same terms in the same order: the results are bit-identical to
+ * lb_eval's.  x, slot: each lane's point and profile slot; f, g: the results
+ * of lane j land in lane j. */
+__device__ __forceinline__ void fit_eval_wide(unsigned long long amask, const double *acc, int slot,
+                              const fsx_fitpar *fp, const double *x, double *f, double *g,
+                              double *sW)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, Q = fp->qnum;
+    const double *qv = fp->q, *av = fp->a, *rdq = fp->rdq;
+    const double mult = fp->mult;
+    while (amask) {
+        const int j = __builtin_ctzll(amask);
+        amask &= amask - 1;
+        const double c1 = lane_d(x[0], j), c2 = lane_d(x[1], j);
+        const double *vp = acc + (long)__builtin_amdgcn_readlane(slot, j) * Q * 6;
+        const double corr = -mult * (c1 * c1 - 1.0), c1c = c1 * c1 * c1;
+        /* scale pass (min_saxs.c:261-319): up, down in lanes 0, 1 */
+        double sum = 0.0, in_last = 0.0;
+        for (int r0 = 0; r0 < Q; r0 += 64) {
+            const int nq = Q - r0 < 64 ? Q - r0 : 64, q = r0 + lane, ok = lane < nq;
+            const int qs = ok ? q : r0;    /* a valid q for the unused lanes */
+            const double2 *v2 = reinterpret_cast<const double2 *>(vp + (long)qs * 6);
+            double v[6];
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const double2 p = v2[h];
+                v[2 * h] = p.x;
+                v[2 * h + 1] = p.y;
+            }
+            const double qc = qv[qs], qp = qs > 0 ? qv[qs - 1] : -1.0;
+            const double G = c1c * exp(corr * qc * qc);
+            const double in = FSX_IN(v, G, c2);
+            double inp = __shfl_up(in, 1);
+            if (lane == 0) inp = r0 == 0 ? in : in_last;
+            double ut, dt;
+            FSX_BS_TERMS(in, inp, qc, qp, av + qs * 6, ut, dt);
+            sW[lane] = ut;
+            sW[64 + lane] = dt;
+            wave_lds_sync();
+            wide_sum(sW, nq, 2, &sum);
+            wave_lds_sync();
+            in_last = lane_d(in, 63);
+        }
+        const double up = lane_d(sum, 0), down = lane_d(sum, 1);
+        const double k = up / down;
+        /* objective (min_saxs.c:3-105): g0, g1, score in lanes 0, 1, 2 */
+        const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
+        double in_l = 0.0, d1_l = 0.0, d2_l = 0.0;
+        sum = 0.0;
+        for (int r0 = 0; r0 < Q; r0 += 64) {
+            const int nq = Q - r0 < 64 ? Q - r0 : 64, q = r0 + lane, ok = lane < nq;
+            const int qs = ok ? q : r0;
+            const double2 *v2 = reinterpret_cast<const double2 *>(vp + (long)qs * 6);
+            double v[6];
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const double2 p = v2[h];
+                v[2 * h] = p.x;
+                v[2 * h + 1] = p.y;
+            }
+            const double qc = qv[qs], qp = qs > 0 ? qv[qs - 1] : -1.0;
+            const double G = c1c * exp(corr * qc * qc);
+            const double Gd = G * (r3c1 - c1m2 * qc * qc);
+            double in, d1, d2;
+            FSX_OBJ_VALS(v, G, Gd, c2, in, d1, d2);
+            double inp = __shfl_up(in, 1), d1p = __shfl_up(d1, 1), d2p = __shfl_up(d2, 1);
+            if (lane == 0) {
+                if (r0 == 0) {
+                    /* the values before the loop (min_saxs.c:28-41) */
+                    inp = FSX_IN(v, G, c2);
+                    d1p = -Gd * v[1] + 2.0 * G * Gd * v[3] - Gd * c2 * v[4];
+                    d2p = v[2] - G * v[4] + 2.0 * c2 * v[5];
+                } else {
+                    inp = in_l;
+                    d1p = d1_l;
+                    d2p = d2_l;
+                }
+            }
+            const double rq = rdq ? rdq[qs] : 1.0 / (qc - qp);
+            double tg0, tg1, tsc;
+            FSX_OBJ_TERMS(in, d1, d2, inp, d1p, d2p, rq, qc, av + qs * 6, k, tg0, tg1, tsc);
+            sW[lane] = tg0;
+            sW[64 + lane] = tg1;
+            sW[128 + lane] = tsc;
+            wave_lds_sync();
+            wide_sum(sW, nq, 3, &sum);
+            wave_lds_sync();
+            in_l = lane_d(in, 63);
+            d1_l = lane_d(d1, 63);
+            d2_l = lane_d(d2, 63);
+        }
+        const double g0 = lane_d(sum, 0), g1 = lane_d(sum, 1), sc = lane_d(sum, 2);
+        if (lane == j) {
+            g[0] = g0;
+            g[1] = g1;
+            *f = sc;
+        }
+    }
+}
 
 template <int AQ_LDS>
 __global__ void __launch_bounds__(256)
 k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
-      double *score, double *c1o, double *c2o, int *next, unsigned long long *tally)
+      double *score, double *c1o, double *c2o, int *next, unsigned long long *tally,
+      int wide_max)
 {
 #pragma clang fp contract(off)
-    /* LDS: the K blocks [FSX_LB_KN][256], then a [6][Q], q [Q] and
+    /* LDS: the K blocks [FSX_LB_KN][256], the wide evaluation's sums
+     * [4 wavefronts][FSX_WIDE_SUMS][64], then a [6][Q], q [Q] and
      * 1/(q_i - q_{i-1}) [Q] when aq_lds (read by every lane at the same
      * address); otherwise a and q are read from global memory and the
      * reciprocals formed per step */
     extern __shared__ double sKf[];
+    double *sW = sKf + FSX_LB_KN * FSX_LB_KSTRIDE + (threadIdx.x >> 6) * FSX_WIDE_SUMS * 64;
     const int nvalid = *nvalid_p;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     fsx_fitpar fp;
     if (AQ_LDS) {
-        double *sA = sKf + FSX_LB_KN * FSX_LB_KSTRIDE, *sQ = sA + 6 * Q, *sR = sQ + Q;
+        double *sA = sKf + FSX_LB_KN * FSX_LB_KSTRIDE + 4 * FSX_WIDE_SUMS * 64;
+        double *sQ = sA + 6 * Q, *sR = sQ + Q;
         for (int i = threadIdx.x; i < 6 * Q; i += blockDim.x) sA[i] = a[i];
         for (int i = threadIdx.x; i < Q; i += blockDim.x) {
             const double qc = qvals[i];
@@ -1089,7 +1244,13 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     int pose = act ? recs[s].x : 0;
     if (act) fit_begin(&t, &S, &fp, acc, s, Q);
     while (__any(act)) {
-        if (act) {
+        /* once the queue is empty and at most wide_max lanes of the
+         * wavefront still fit, the wavefront evaluates their fits together
+         * (q across the lanes) instead of each lane running its whole q loop */
+        const unsigned long long amask = __ballot(act);
+        if (__popcll(amask) <= wide_max) {
+            fit_eval_wide(amask, acc, s, &fp, S.x, &S.f, S.g, sW);
+        } else if (act) {
             double k;
             lb_eval(&t, &fp, S.x, &S.f, S.g, &k);
         }
@@ -1643,7 +1804,7 @@ static const int k_plan_slot[P_NBUF] = {
 
 typedef void (*fit_kernel_t)(const double *, const int4 *, const int *, int, const double *,
                              const double *, double, double, double, double, double, double *,
-                             double *, double *, int *, unsigned long long *);
+                             double *, double *, int *, unsigned long long *, int);
 
 struct fsx_plan {
     fsx_hip_args A;
@@ -1661,6 +1822,8 @@ struct fsx_plan {
     size_t lds_fit;
     long lanes;
     fit_kernel_t kfit;
+    int fit_wide;        /* lanes with a fit at or below which a fit wavefront
+                            evaluates them together */
     int tile_rows;       /* gamma2 rows per cross-term tile (0: the form's default) */
     int rot_gt;          /* gamma per block of the MFMA rotation (16 or 8), 0: the
                             rotation-then-transpose kernels */
@@ -1820,6 +1983,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     p->aq_lds = fit_aq_lds(Q);
     p->lds_fit = fit_lds(Q);
     p->kfit = p->aq_lds ? k_fit<1> : k_fit<0>;
+    p->fit_wide = env_int("FSX_FIT_WIDE", 16, 0, 64);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, p->kfit, 256,
                                                                       p->lds_fit) != hipSuccess ||
@@ -2021,7 +2185,8 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
                        (const double *)g[P_ACC], (const int4 *)g[P_RECS],
                        (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
                        (const double *)g[P_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo, A->c2_hi,
-                       p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT], (unsigned long long *)g[P_TALLY]);
+                       p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT], (unsigned long long *)g[P_TALLY],
+                       p->fit_wide);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[6], s));
     return FSX_OK;

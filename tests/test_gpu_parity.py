@@ -97,7 +97,7 @@ def test_native_grid_vs_oracle(F, L, Q, qmax, nat):
 def test_many_q_vs_oracle(F):
     """400 q points: the moments, the q grid and the reciprocals of the
     q steps no longer fit the fit kernel's LDS beside the L-BFGS-B blocks
-    (at most 256 q do), so the fits read them from global memory; the
+    (at most 160 q do), so the fits read them from global memory; the
     reference has no limit on the number of q points."""
     L = 6
     zv = np.arange(20.0, 26.0)
