@@ -355,7 +355,13 @@ def main():
                     "unit": "GB/s", "frac": achieved / 8000.0, "traffic": traffic.get("k_fit"),
                     "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb,
                     "evaluations_per_fit": float(np.mean([s["fit_evals"] for s in stats]))
-                    / max(1, stats[-1]["n_scored"])}
+                    / max(1, stats[-1]["n_scored"]),
+                    # loop iterations of the fit kernel: summed over its wavefronts, of
+                    # those wavefront-wide (the last few fits of a wavefront evaluated
+                    # together), and the longest-running wavefront's
+                    "wavefront_steps": float(np.mean([s["fit_steps"] for s in stats])),
+                    "wide_steps": float(np.mean([s["fit_wide_steps"] for s in stats])),
+                    "longest_wavefront_steps": float(np.mean([s["fit_max_steps"] for s in stats]))}
         # the longer of the two is the dominant kernel; the other is reported beside
         roof, other = ((roof_cross, roof_fit) if avg["ms_cross"] >= avg["ms_fit"]
                        else (roof_fit, roof_cross))

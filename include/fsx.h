@@ -91,6 +91,11 @@ typedef struct fsx_stats {
                             (L+1)(L+2)/2 when both molecules' coefficients are
                             conjugate symmetric and only m >= 0 is formed */
     double ms_fit_prep;  /* profile rescale (peak / I(q0)) before the fits (GPU) */
+    long fit_steps;      /* fit loop iterations (one evaluation and one optimiser advance
+                            each) summed over the fit kernel's wavefronts */
+    long fit_wide_steps; /* of those, iterations in which the wavefront evaluated its last
+                            few fits together (q across the lanes) */
+    long fit_max_steps;  /* iterations of the longest-running fit wavefront */
 } fsx_stats;
 
 /* Score n poses.  score/c1/c2 receive one value per pose whose z slot is

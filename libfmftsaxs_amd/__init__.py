@@ -41,7 +41,8 @@ class Stats(C.Structure):
                 ("ms_cross", C.c_double), ("ms_fit", C.c_double), ("ms_total", C.c_double),
                 ("cross_bytes", C.c_double), ("cross_flops", C.c_double),
                 ("cross_items", C.c_long), ("cross_chunks", C.c_int), ("cross_entries", C.c_int),
-                ("ms_fit_prep", C.c_double)]
+                ("ms_fit_prep", C.c_double), ("fit_steps", C.c_long),
+                ("fit_wide_steps", C.c_long), ("fit_max_steps", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

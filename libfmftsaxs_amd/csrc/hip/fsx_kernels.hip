@@ -37,6 +37,9 @@
 #define FSX_CROSS_THREADS 256
 #define FSX_WAVE 64
 #define FSX_NXCD 8
+/* counters of the fit kernel: evaluations, capped fits, loop iterations summed
+ * over wavefronts, of those wavefront-wide ones, the longest wavefront's */
+#define FSX_TALLY_N 5
 
 struct cplx { double re, im; };
 struct cplx32 { float re, im; };   /* element of the mixed-precision (FP32) tables */
@@ -1107,6 +1110,79 @@ __device__ __forceinline__ void fit_eval_wide(unsigned long long amask, const do
     const int lane = threadIdx.x & 63, Q = fp->qnum;
     const double *qv = fp->q, *av = fp->a, *rdq = fp->rdq;
     const double mult = fp->mult;
+    if (Q <= 64) {
+        /* one round: each lane's q, its neighbour and 1/(q - q_prev) once;
+         * the profile of a fit read once for both passes (the next fit's
+         * read issued before this one's sums); G and I of the scale pass
+         * reused by the objective */
+        const int qs = lane < Q ? lane : 0;
+        const double qc = qv[qs], qp = qs > 0 ? qv[qs - 1] : -1.0;
+        const double rq = rdq ? rdq[qs] : 1.0 / (qc - qp);
+        const double *ai = av + qs * 6;
+        double vn[6];
+        auto load = [&](int jj) {
+            const double2 *v2 = reinterpret_cast<const double2 *>(
+                acc + ((long)__builtin_amdgcn_readlane(slot, jj) * Q + qs) * 6);
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const double2 p = v2[h];
+                vn[2 * h] = p.x;
+                vn[2 * h + 1] = p.y;
+            }
+        };
+        load(__builtin_ctzll(amask));
+        while (amask) {
+            const int j = __builtin_ctzll(amask);
+            amask &= amask - 1;
+            double v[6];
+#pragma unroll
+            for (int h = 0; h < 6; h++) v[h] = vn[h];
+            if (amask) load(__builtin_ctzll(amask));
+            const double c1 = lane_d(x[0], j), c2 = lane_d(x[1], j);
+            const double corr = -mult * (c1 * c1 - 1.0), c1c = c1 * c1 * c1;
+            const double G = c1c * exp(corr * qc * qc);
+            const double in = FSX_IN(v, G, c2);
+            double inp = __shfl_up(in, 1);
+            if (lane == 0) inp = in;
+            double ut, dt;
+            FSX_BS_TERMS(in, inp, qc, qp, ai, ut, dt);
+            sW[lane] = ut;
+            sW[64 + lane] = dt;
+            wave_lds_sync();
+            double sum = 0.0;
+            wide_sum(sW, Q, 2, &sum);
+            wave_lds_sync();
+            const double k = lane_d(sum, 0) / lane_d(sum, 1);
+            const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
+            const double Gd = G * (r3c1 - c1m2 * qc * qc);
+            double in2, d1, d2;
+            FSX_OBJ_VALS(v, G, Gd, c2, in2, d1, d2);
+            double d1p = __shfl_up(d1, 1), d2p = __shfl_up(d2, 1);
+            inp = __shfl_up(in2, 1);
+            if (lane == 0) {
+                /* the values before the loop (min_saxs.c:28-41) */
+                inp = FSX_IN(v, G, c2);
+                d1p = -Gd * v[1] + 2.0 * G * Gd * v[3] - Gd * c2 * v[4];
+                d2p = v[2] - G * v[4] + 2.0 * c2 * v[5];
+            }
+            double tg0, tg1, tsc;
+            FSX_OBJ_TERMS(in2, d1, d2, inp, d1p, d2p, rq, qc, ai, k, tg0, tg1, tsc);
+            sW[lane] = tg0;
+            sW[64 + lane] = tg1;
+            sW[128 + lane] = tsc;
+            wave_lds_sync();
+            sum = 0.0;
+            wide_sum(sW, Q, 3, &sum);
+            wave_lds_sync();
+            const double g0 = lane_d(sum, 0), g1 = lane_d(sum, 1), sc = lane_d(sum, 2);
+            if (lane == j) {
+                g[0] = g0;
+                g[1] = g1;
+                *f = sc;
+            }
+        }
+        return;
+    }
     while (amask) {
         const int j = __builtin_ctzll(amask);
         amask &= amask - 1;
@@ -1239,7 +1315,7 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     t.scale = 1.0;
     fsx_lbrc S;
     S.w.kb = sKf + threadIdx.x;
-    unsigned long long nev = 0, ncap = 0;
+    unsigned long long nev = 0, ncap = 0, steps = 0, wsteps = 0;
     int act = s < nvalid;
     int pose = act ? recs[s].x : 0;
     if (act) fit_begin(&t, &S, &fp, acc, s, Q);
@@ -1248,7 +1324,9 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
          * wavefront still fit, the wavefront evaluates their fits together
          * (q across the lanes) instead of each lane running its whole q loop */
         const unsigned long long amask = __ballot(act);
+        steps++;
         if (__popcll(amask) <= wide_max) {
+            wsteps++;
             fit_eval_wide(amask, acc, s, &fp, S.x, &S.f, S.g, sW);
         } else if (act) {
             double k;
@@ -1270,6 +1348,11 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     }
     if (nev) atomicAdd(&tally[0], nev);
     if (ncap) atomicAdd(&tally[1], ncap);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&tally[2], steps);
+        atomicAdd(&tally[3], wsteps);
+        atomicMax(&tally[4], steps);
+    }
 }
 
 /* fits of caller-supplied (rescaled) profiles ct[n][6][Q] */
@@ -1973,7 +2056,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     z[P_TW2] = sizeof(double) * 2 * (size_t)na * N;
     z[P_COFF] = sizeof(long) * (C + 1);
     z[P_NEXT] = sizeof(int);
-    z[P_TALLY] = 2 * sizeof(unsigned long long);
+    z[P_TALLY] = FSX_TALLY_N * sizeof(unsigned long long);
     z[P_SRCT] = sizeof(cplx) * (size_t)Q * 3 * K2 * ng;
     z[P_POSES] = (A->flags & FSX_DEVICE_POSES) ? 0 : sizeof(long long) * n;
     z[P_SCORE] = z[P_C1] = z[P_C2] = p->dev_out ? 0 : sizeof(double) * n;
@@ -1983,7 +2066,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     p->aq_lds = fit_aq_lds(Q);
     p->lds_fit = fit_lds(Q);
     p->kfit = p->aq_lds ? k_fit<1> : k_fit<0>;
-    p->fit_wide = env_int("FSX_FIT_WIDE", 16, 0, 64);
+    p->fit_wide = env_int("FSX_FIT_WIDE", 24, 0, 64);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, p->kfit, 256,
                                                                       p->lds_fit) != hipSuccess ||
@@ -2170,7 +2253,7 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
     if (ev) HIPR(hipEventRecord(ev[4], s));
     if (stop_after_cross) return FSX_OK;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_NEXT], 1L, (int)p->lanes);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_TALLY], 4L, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_TALLY], 2L * FSX_TALLY_N, 0);
     /* rescaled profiles in place of the pose parts (bounded by n slots;
      * the kernels read the number of valid slots on the device) */
     hipLaunchKernelGGL(k_fit_scale, dim3(nblk(n, 256)), dim3(256), 0, s, (const double *)g[P_ACC],
@@ -2207,7 +2290,8 @@ static int plan_download(fsx_plan *p, hipStream_t s, int *tail, unsigned long lo
                                 hipMemcpyDeviceToHost, s));
     }
     HIPR(hipMemcpyAsync(tail, (int *)p->d[P_OFFS] + p->nkeys, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-    HIPR(hipMemcpyAsync(tally, p->d[P_TALLY], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPR(hipMemcpyAsync(tally, p->d[P_TALLY], FSX_TALLY_N * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s));
     return FSX_OK;
 }
 
@@ -2230,6 +2314,9 @@ static void plan_stats(const fsx_plan *p, fsx_stats *st, const float *ms, const 
     st->n_scored = nvalid;
     st->fit_evals = (long)tally[0];
     st->n_capped = (int)tally[1];
+    st->fit_steps = (long)tally[2];
+    st->fit_wide_steps = (long)tally[3];
+    st->fit_max_steps = (long)tally[4];
     st->cross_items = items;
     st->cross_chunks = C;
     /* useful FP64 flops of the cross-term launches: 8 per complex
@@ -2271,7 +2358,7 @@ static int run_locked(const fsx_hip_args *A, fsx_stats *st)
     hipEvent_t ev[7];
     int nev_made = 0, rc = FSX_OK;
     int tail[2] = {0, 0};
-    unsigned long long tally[2] = {0, 0};
+    unsigned long long tally[FSX_TALLY_N] = {0};
     float ms[6];
     rc = plan_bind(&p, get_slot, NULL);
     if (rc == FSX_OK) {
@@ -2542,7 +2629,8 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
     }
     fsx_plan *pl = (fsx_plan *)calloc(nprob, sizeof(fsx_plan));
     int *tail = (int *)calloc(2 * (size_t)nprob, sizeof(int));
-    unsigned long long *tally = (unsigned long long *)calloc(2 * (size_t)nprob, sizeof(unsigned long long));
+    unsigned long long *tally =
+        (unsigned long long *)calloc(FSX_TALLY_N * (size_t)nprob, sizeof(unsigned long long));
     fsx_gkey *key = (fsx_gkey *)calloc(nprob > 0 ? nprob : 1, sizeof(fsx_gkey));
     int rc = (pl && tail && tally && key) ? FSX_OK : FSX_ERR_NOMEM;
     int ninit = 0;
@@ -2632,10 +2720,10 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
             rc = FSX_ERR_HIP;
     }
     for (int i = 0; rc == FSX_OK && i < nprob; i++)
-        rc = plan_download(&pl[i], g_stream, tail + 2 * i, tally + 2 * i);
+        rc = plan_download(&pl[i], g_stream, tail + 2 * i, tally + FSX_TALLY_N * i);
     if (hipStreamSynchronize(g_stream) != hipSuccess && rc == FSX_OK) rc = FSX_ERR_HIP;
     for (int i = 0; rc == FSX_OK && i < nprob; i++)
-        plan_stats(&pl[i], &st[i], NULL, tail + 2 * i, tally + 2 * i);
+        plan_stats(&pl[i], &st[i], NULL, tail + 2 * i, tally + FSX_TALLY_N * i);
     if (rc != FSX_OK) graph_drop();
     for (int i = 0; i < ninit; i++) plan_free_host(&pl[i]);
     free(pl);
