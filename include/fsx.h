@@ -125,6 +125,15 @@ int fsx_fit_host(const double *ct, const double *qvals, int qnum, const double *
                  double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
                  double *out4);
 
+/* fsx_fit_host in either form: flags 0 (the reference's evaluation order,
+ * iterates bit-identical to sxs_lbfgs_fitting) or FSX_MIXED_FP32 (the
+ * mixed-precision form's fit: scale, objective and gradient from one pass
+ * over q, equal to the two-pass values up to rounding).  No counterpart in
+ * the reference; it lets a caller reproduce either form's fit on the host. */
+int fsx_fit_host_form(const double *ct, const double *qvals, int qnum, const double *a,
+                      double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                      unsigned flags, double *out4);
+
 /* Fit many profiles on the GPU: ct [n][6][qnum] (host), already rescaled.
  * out [n][4]: score, c1, c2, evaluations. */
 int fsx_fit_batch(const double *ct, int n, const double *qvals, int qnum, const double *a,

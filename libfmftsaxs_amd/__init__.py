@@ -91,6 +91,9 @@ def lib():
         L.fsx_score_batch.restype = C.c_int
         L.fsx_fit_host.argtypes = [_dp, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
         L.fsx_fit_host.restype = C.c_int
+        L.fsx_fit_host_form.argtypes = [_dp, _dp, C.c_int, _dp] + [C.c_double] * 5 + [C.c_uint,
+                                                                                       _dp]
+        L.fsx_fit_host_form.restype = C.c_int
         L.fsx_fit_batch.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp] + [C.c_double] * 5 + [_dp]
         L.fsx_fit_batch.restype = C.c_int
         L.fsx_sbessel_batch.argtypes = [_dp, C.c_int, C.c_int, _dp]
@@ -252,11 +255,13 @@ def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0, mixed=Fals
     return st.as_dict()
 
 
-def fit_host(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0)):
+def fit_host(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0), mixed=False):
+    """Host fit of one rescaled profile ct [6,Q] -> (score, c1, c2, evals); mixed: the
+    mixed-precision form's one-pass evaluation instead of the reference's two passes."""
     ct = f64(ct)
     out = np.zeros(4)
-    _chk(lib().fsx_fit_host(ct.reshape(-1), f64(qvals), ct.shape[-1], f64(a), mult, c1[0],
-                            c1[1], c2[0], c2[1], out))
+    _chk(lib().fsx_fit_host_form(ct.reshape(-1), f64(qvals), ct.shape[-1], f64(a), mult,
+                                 c1[0], c1[1], c2[0], c2[1], FSX_MIXED_FP32 if mixed else 0, out))
     return out
 
 

@@ -130,6 +130,10 @@ typedef struct fsx_fitpar {
     double mult;          /* (4pi/3)^1.5 rm^2 / (16 pi) */
     double lo0, hi0, lo1, hi1;
     double factr, pgtol;
+    /* onepass (the mixed-precision form): the evaluation by fsx_eval_1p, in
+     * one pass over q, with a0sum = the sum of a[6 i] over q in q order */
+    int onepass;
+    double a0sum;
 } fsx_fitpar;
 
 /* The six terms at q index i, each (c + v) * scale as fsx_ct_get. */
@@ -318,6 +322,47 @@ FSX_HD double fsx_in(const fsx_ct *t, int i, double G, double c2)
                                           + tan_ * tan_ * (ai)[5]));                  \
     } while (0)
 
+/* FSX_1P_TERMS: the six terms one q step adds to the sums of the one-pass
+ * evaluation (fsx_eval_1p), from the values at this q and the previous one
+ * (as FSX_OBJ_TERMS takes them).  With tan, t_j the slopes of I and of its
+ * derivative d_j in c_j over the step (the same slope serves the scale pass,
+ * min_saxs.c:302-303), b = I - tan q and e_j = d_j - t_j q, the reference's
+ * per-q sums (min_saxs.c:70-93, 302-310) are
+ *   up    = sum (b a1 + tan a2)
+ *   down  = sum (b^2 a3 + 2 tan b a4 + tan^2 a5)
+ *   score = sum a0 - 2 k up + k^2 down
+ *   g_j   = 2 k sum (-e_j a1 - t_j a2)
+ *         + 2 k^2 sum (b e_j a3 + (I t_j + d_j tan - 2 tan t_j q) a4 + tan t_j a5)
+ * with k = up / down.  None of the six sums depends on k, so one pass over
+ * q forms all of them. */
+#define FSX_1P_TERMS(in, d1, d2, in_prev, d1p, d2p, rq, qc, ai, tt)                   \
+    do {                                                                              \
+        const double tan_ = ((in) - (in_prev)) * (rq);                                \
+        const double t1_ = ((d1) - (d1p)) * (rq);                                     \
+        const double t2_ = ((d2) - (d2p)) * (rq);                                     \
+        const double b_ = (in) - tan_ * (qc);                                         \
+        const double e1_ = (d1) - t1_ * (qc), e2_ = (d2) - t2_ * (qc);                \
+        (tt)[0] = b_ * (ai)[1] + tan_ * (ai)[2];                                      \
+        (tt)[1] = b_ * b_ * (ai)[3] + 2.0 * tan_ * b_ * (ai)[4] + tan_ * tan_ * (ai)[5]; \
+        (tt)[2] = -e1_ * (ai)[1] - t1_ * (ai)[2];                                     \
+        (tt)[3] = b_ * e1_ * (ai)[3]                                                  \
+                + ((in) * t1_ + (d1) * tan_ - 2.0 * tan_ * t1_ * (qc)) * (ai)[4]      \
+                + tan_ * t1_ * (ai)[5];                                               \
+        (tt)[4] = -e2_ * (ai)[1] - t2_ * (ai)[2];                                     \
+        (tt)[5] = b_ * e2_ * (ai)[3]                                                  \
+                + ((in) * t2_ + (d2) * tan_ - 2.0 * tan_ * t2_ * (qc)) * (ai)[4]      \
+                + tan_ * t2_ * (ai)[5];                                               \
+    } while (0)
+
+/* the value and gradient from the six one-pass sums s and sum a0 */
+FSX_HD void fsx_1p_finish(const double *s, double a0sum, double *fval, double *g)
+{
+    const double k = s[0] / s[1];
+    *fval = a0sum + k * (-2.0 * s[0] + k * s[1]);
+    g[0] = 2.0 * k * (s[2] + k * s[3]);
+    g[1] = 2.0 * k * (s[4] + k * s[5]);
+}
+
 /* min_saxs.c:261-319.  bA holds the first block of the profile (q = 0 ..
  * FSX_QB-1) on entry; it is overwritten. */
 FSX_HD double fsx_best_scale_blk(const fsx_ct *t, const fsx_fitpar *p, double c1, double c2,
@@ -404,6 +449,56 @@ FSX_HD void fsx_objective(const fsx_ct *t, const fsx_fitpar *p, double k,
     double bA[FSX_QB][6];
     fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
     fsx_objective_blk(t, p, k, x, g, fval, bA);
+}
+
+/* The evaluation of the mixed-precision form: the scale, the objective and
+ * its gradient (min_saxs.c:3-105, 261-319) from one pass over q, by the sums
+ * of FSX_1P_TERMS: one read of the profile and one exp per q instead of two.
+ * The values equal the two-pass ones up to rounding, so the fits end at the
+ * reference's optimum within its stopping tolerance, not on its exact
+ * iterates.  bA holds the first block of the profile on entry. */
+FSX_HD void fsx_eval_1p_blk(const fsx_ct *t, const fsx_fitpar *p, const double *x, double *g,
+                            double *fval, double bA[FSX_QB][6])
+{
+    const double *a = p->a, *q = p->q, *rdq = p->rdq;
+    const int qnum = p->qnum;
+    double v0[6];
+    for (int j = 0; j < 6; j++) v0[j] = bA[0][j];
+    const double c1 = x[0], c2 = x[1], mult = p->mult;
+    const double corr = -mult * (c1 * c1 - 1.0);
+    const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
+    double G = c1 * c1 * c1 * exp(corr * q[0] * q[0]);
+    double Gd = G * (r3c1 - c1m2 * q[0] * q[0]);
+    double in_prev, d1p, d2p;
+    FSX_OBJ_VALS(v0, G, Gd, c2, in_prev, d1p, d2p);
+    double q_prev = -1.0, c1c = c1 * c1 * c1;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#define FSX_1P_STEP(cur, i)                                                           \
+    do {                                                                              \
+        double qc = q[i];                                                             \
+        G = c1c * exp(corr * qc * qc);                                                \
+        Gd = G * (r3c1 - c1m2 * qc * qc);                                             \
+        double in, d1, d2, tt[6];                                                     \
+        FSX_OBJ_VALS(cur, G, Gd, c2, in, d1, d2);                                     \
+        const double rq = rdq ? rdq[i] : 1.0 / (qc - q_prev);                         \
+        FSX_1P_TERMS(in, d1, d2, in_prev, d1p, d2p, rq, qc, a + (i) * 6, tt);         \
+        for (int j_ = 0; j_ < 6; j_++) s[j_] += tt[j_];                               \
+        in_prev = in;                                                                 \
+        d1p = d1;                                                                     \
+        d2p = d2;                                                                     \
+        q_prev = qc;                                                                  \
+    } while (0)
+    FSX_QLOOP(t, qnum, bA, FSX_1P_STEP);
+#undef FSX_1P_STEP
+    fsx_1p_finish(s, p->a0sum, fval, g);
+}
+
+/* sum of a[6 i] over q, in q order (the constant part of the objective) */
+FSX_HD double fsx_a0sum(const double *a, int qnum)
+{
+    double s = 0.0;
+    for (int i = 0; i < qnum; i++) s += a[6 * i];
+    return s;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1659,6 +1754,14 @@ FSX_HD void lb_eval(const fsx_ct *t, const fsx_fitpar *p, const double *x, doubl
     fsx_objective_blk(t, p, k, x, g, f, bA);
 }
 
+FSX_HD void lb_eval_1p(const fsx_ct *t, const fsx_fitpar *p, const double *x, double *f,
+                       double *g)
+{
+    double bA[FSX_QB][6];
+    fsx_ct_block(t, 0, FSX_QB_MIN(p->qnum, FSX_QB), bA);
+    fsx_eval_1p_blk(t, p, x, g, f, bA);
+}
+
 /* ---------------------------------------------------------------------- */
 /* Resumable driver.  setulb's reverse communication (lbfgsb.c:52-515, driven
  * by min_saxs.c:234-248) is kept as an explicit state machine:
@@ -1879,7 +1982,10 @@ FSX_HD int fsx_lbfgsb_fit(const fsx_ct *ct, const fsx_fitpar *p, double *x, doub
     FSX_LB_KSET(S);
     fsx_lb_start(&S, p, x[0], x[1]);
     do {
-        lb_eval(ct, p, S.x, &S.f, S.g, &k);
+        if (p->onepass)
+            lb_eval_1p(ct, p, S.x, &S.f, S.g);
+        else
+            lb_eval(ct, p, S.x, &S.f, S.g, &k);
     } while (fsx_lb_resume(&S));
     x[0] = S.x[0];
     x[1] = S.x[1];

@@ -1093,6 +1093,131 @@ __device__ __forceinline__ void wide_sum(const double *sW, int nq, int nc, doubl
     }
 }
 
+/* The six one-pass sums of the wide evaluation, two rows of three at a
+ * time: terms tt[0..5] of this lane's q (rows 0..nq-1 used), added in q
+ * order into sa (sums 0..2, lanes 0..2) and sb (sums 3..5, lanes 0..2). */
+__device__ __forceinline__ void wide_sum6(double *sW, const double *tt, int nq, double *sa,
+                                          double *sb)
+{
+    const int lane = threadIdx.x & 63;
+    sW[lane] = tt[0];
+    sW[64 + lane] = tt[1];
+    sW[128 + lane] = tt[2];
+    wave_lds_sync();
+    wide_sum(sW, nq, 3, sa);
+    wave_lds_sync();
+    sW[lane] = tt[3];
+    sW[64 + lane] = tt[4];
+    sW[128 + lane] = tt[5];
+    wave_lds_sync();
+    wide_sum(sW, nq, 3, sb);
+    wave_lds_sync();
+}
+
+/* The wide evaluation of the mixed-precision form's one-pass objective
+ * (FSX_1P_TERMS): the profile of a fit is read once per evaluation and each
+ * q's G is formed once.  Lane i holds q = 64 r + i in round r; the sums add
+ * the terms in q order, as fsx_eval_1p_blk does, so the results are
+ * bit-identical to the lane-per-fit evaluation's. */
+__device__ __forceinline__ void fit_eval_wide_1p(unsigned long long amask, const double *acc,
+                                                 int slot, const fsx_fitpar *fp, const double *x,
+                                                 double *f, double *g, double *sW)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, Q = fp->qnum;
+    const double *qv = fp->q, *av = fp->a, *rdq = fp->rdq;
+    const double mult = fp->mult;
+    if (Q <= 64) {
+        const int qs = lane < Q ? lane : 0;
+        const double qc = qv[qs], qp = qs > 0 ? qv[qs - 1] : -1.0;
+        const double rq = rdq ? rdq[qs] : 1.0 / (qc - qp);
+        const double *ai = av + qs * 6;
+        double vn[6];
+        auto load = [&](int jj) {
+            const double2 *v2 = reinterpret_cast<const double2 *>(
+                acc + ((long)__builtin_amdgcn_readlane(slot, jj) * Q + qs) * 6);
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const double2 p = v2[h];
+                vn[2 * h] = p.x;
+                vn[2 * h + 1] = p.y;
+            }
+        };
+        load(__builtin_ctzll(amask));
+        while (amask) {
+            const int j = __builtin_ctzll(amask);
+            amask &= amask - 1;
+            double v[6];
+#pragma unroll
+            for (int h = 0; h < 6; h++) v[h] = vn[h];
+            if (amask) load(__builtin_ctzll(amask));
+            const double c1 = lane_d(x[0], j), c2 = lane_d(x[1], j);
+            const double corr = -mult * (c1 * c1 - 1.0), c1c = c1 * c1 * c1;
+            const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
+            const double G = c1c * exp(corr * qc * qc);
+            const double Gd = G * (r3c1 - c1m2 * qc * qc);
+            double in, d1, d2;
+            FSX_OBJ_VALS(v, G, Gd, c2, in, d1, d2);
+            double inp = __shfl_up(in, 1), d1p = __shfl_up(d1, 1), d2p = __shfl_up(d2, 1);
+            if (lane == 0) {
+                inp = in;
+                d1p = d1;
+                d2p = d2;
+            }
+            double tt[6];
+            FSX_1P_TERMS(in, d1, d2, inp, d1p, d2p, rq, qc, ai, tt);
+            double sa = 0.0, sb = 0.0;
+            wide_sum6(sW, tt, Q, &sa, &sb);
+            const double sm[6] = {lane_d(sa, 0), lane_d(sa, 1), lane_d(sa, 2),
+                                  lane_d(sb, 0), lane_d(sb, 1), lane_d(sb, 2)};
+            if (lane == j) fsx_1p_finish(sm, fp->a0sum, f, g);
+        }
+        return;
+    }
+    while (amask) {
+        const int j = __builtin_ctzll(amask);
+        amask &= amask - 1;
+        const double c1 = lane_d(x[0], j), c2 = lane_d(x[1], j);
+        const double *vp = acc + (long)__builtin_amdgcn_readlane(slot, j) * Q * 6;
+        const double corr = -mult * (c1 * c1 - 1.0), c1c = c1 * c1 * c1;
+        const double r3c1 = 3.0 / c1, c1m2 = 2.0 * c1 * mult;
+        double in_l = 0.0, d1_l = 0.0, d2_l = 0.0, sa = 0.0, sb = 0.0;
+        for (int r0 = 0; r0 < Q; r0 += 64) {
+            const int nq = Q - r0 < 64 ? Q - r0 : 64, q = r0 + lane, ok = lane < nq;
+            const int qs = ok ? q : r0;    /* a valid q for the unused lanes */
+            const double2 *v2 = reinterpret_cast<const double2 *>(vp + (long)qs * 6);
+            double v[6];
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const double2 p = v2[h];
+                v[2 * h] = p.x;
+                v[2 * h + 1] = p.y;
+            }
+            const double qc = qv[qs], qp = qs > 0 ? qv[qs - 1] : -1.0;
+            const double G = c1c * exp(corr * qc * qc);
+            const double Gd = G * (r3c1 - c1m2 * qc * qc);
+            double in, d1, d2;
+            FSX_OBJ_VALS(v, G, Gd, c2, in, d1, d2);
+            double inp = __shfl_up(in, 1), d1p = __shfl_up(d1, 1), d2p = __shfl_up(d2, 1);
+            if (lane == 0) {
+                inp = r0 == 0 ? in : in_l;
+                d1p = r0 == 0 ? d1 : d1_l;
+                d2p = r0 == 0 ? d2 : d2_l;
+            }
+            const double rq = rdq ? rdq[qs] : 1.0 / (qc - qp);
+            double tt[6];
+            FSX_1P_TERMS(in, d1, d2, inp, d1p, d2p, rq, qc, av + qs * 6, tt);
+            wide_sum6(sW, tt, nq, &sa, &sb);
+            in_l = lane_d(in, 63);
+            d1_l = lane_d(d1, 63);
+            d2_l = lane_d(d2, 63);
+        }
+        const double sm[6] = {lane_d(sa, 0), lane_d(sa, 1), lane_d(sa, 2),
+                              lane_d(sb, 0), lane_d(sb, 1), lane_d(sb, 2)};
+        if (lane == j) fsx_1p_finish(sm, fp->a0sum, f, g);
+    }
+}
+
 /* Evaluation of the fits of the lanes in amask (all of this wavefront) by
  * the whole wavefront, one fit at a time: lane i holds q = 64 r + i in
  * round r, forms that q's terms (FSX_BS_TERMS, FSX_OBJ_VALS, FSX_OBJ_TERMS,
@@ -1102,6 +1227,7 @@ __device__ __forceinline__ void wide_sum(const double *sW, int nq, int nc, doubl
  * adds the same terms in the same order: the results are bit-identical to
  * lb_eval's.  x, slot: each lane's point and profile slot; f, g: the results
  * of lane j land in lane j. */
+template <int ONEP>
 __device__ __forceinline__ void fit_eval_wide(unsigned long long amask, const double *acc, int slot,
                               const fsx_fitpar *fp, const double *x, double *f, double *g,
                               double *sW)
@@ -1110,6 +1236,10 @@ __device__ __forceinline__ void fit_eval_wide(unsigned long long amask, const do
     const int lane = threadIdx.x & 63, Q = fp->qnum;
     const double *qv = fp->q, *av = fp->a, *rdq = fp->rdq;
     const double mult = fp->mult;
+    if (ONEP) {
+        fit_eval_wide_1p(amask, acc, slot, fp, x, f, g, sW);
+        return;
+    }
     if (Q <= 64) {
         /* one round: each lane's q, its neighbour and 1/(q - q_prev) once;
          * the profile of a fit read once for both passes (the next fit's
@@ -1273,7 +1403,7 @@ __device__ __forceinline__ void fit_eval_wide(unsigned long long amask, const do
     }
 }
 
-template <int AQ_LDS>
+template <int AQ_LDS, int ONEP>
 __global__ void __launch_bounds__(256)
 k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
@@ -1308,6 +1438,8 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
+    fp.onepass = ONEP;
+    fp.a0sum = ONEP ? fsx_a0sum(fp.a, Q) : 0.0;
     /* profiles already rescaled in place (k_fit_prep) */
     fsx_ct t;
     t.c = NULL; t.cks = 0; t.cqs = 0;
@@ -1327,10 +1459,14 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
         steps++;
         if (__popcll(amask) <= wide_max) {
             wsteps++;
-            fit_eval_wide(amask, acc, s, &fp, S.x, &S.f, S.g, sW);
+            fit_eval_wide<ONEP>(amask, acc, s, &fp, S.x, &S.f, S.g, sW);
         } else if (act) {
-            double k;
-            lb_eval(&t, &fp, S.x, &S.f, S.g, &k);
+            if (ONEP) {
+                lb_eval_1p(&t, &fp, S.x, &S.f, S.g);
+            } else {
+                double k;
+                lb_eval(&t, &fp, S.x, &S.f, S.g, &k);
+            }
         }
         if (act && !fsx_lb_resume(&S)) {
             score[pose] = sqrt(S.f);
@@ -1371,6 +1507,7 @@ k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a
     fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = Q; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
+    fp.onepass = 0; fp.a0sum = 0.0;
     double x[2] = {1.0, 0.0}, f;
     int nev;
     fsx_lbfgsb_fit(&t, &fp, x, &f, &nev, sKb + threadIdx.x);
@@ -2065,7 +2202,9 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
      * state lives in registers, one 256-thread block per CU) */
     p->aq_lds = fit_aq_lds(Q);
     p->lds_fit = fit_lds(Q);
-    p->kfit = p->aq_lds ? k_fit<1> : k_fit<0>;
+    /* the mixed-precision form's fit evaluates in one pass over q */
+    p->kfit = p->aq_lds ? (p->mixed ? k_fit<1, 1> : k_fit<1, 0>)
+                        : (p->mixed ? k_fit<0, 1> : k_fit<0, 0>);
     p->fit_wide = env_int("FSX_FIT_WIDE", 24, 0, 64);
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, p->kfit, 256,

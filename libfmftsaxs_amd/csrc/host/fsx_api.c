@@ -342,6 +342,8 @@ static fsx_fitpar fitpar_of(const struct sxs_profile *pr, const struct sxs_opt_p
     fp.lo0 = 0.96; fp.hi0 = 1.04; fp.lo1 = -2.00; fp.hi1 = 4.00;   /* define.h:28-31 */
     fp.factr = 1e+7;
     fp.pgtol = 1e-5;
+    fp.onepass = 0;
+    fp.a0sum = 0.0;
     return fp;
 }
 
@@ -459,12 +461,21 @@ int fsx_fit_host(const double *ct, const double *qvals, int qnum, const double *
                  double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
                  double *out4)
 {
+    return fsx_fit_host_form(ct, qvals, qnum, a, mult, c1_lo, c1_hi, c2_lo, c2_hi, 0u, out4);
+}
+
+int fsx_fit_host_form(const double *ct, const double *qvals, int qnum, const double *a,
+                      double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+                      unsigned flags, double *out4)
+{
     if (!ct || !qvals || !a || !out4 || qnum < 2) return FSX_ERR_ARG;
     fsx_ct t = ct_packed(ct, qnum);
     fsx_fitpar fp;
     fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = qnum; fp.mult = mult;
     fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
     fp.factr = 1e+7; fp.pgtol = 1e-5;
+    fp.onepass = (flags & FSX_MIXED_FP32) != 0;
+    fp.a0sum = fsx_a0sum(a, qnum);
     double x[2] = {1.0, 0.0}, f;
     int nev;
     fsx_lbfgsb_fit(&t, &fp, x, &f, &nev);

@@ -97,6 +97,7 @@ int fsx_score_naive(int nrec, int nlig, const double *xyz, const double *vac,
         fp.a = a; fp.q = qvals; fp.rdq = NULL; fp.qnum = qnum; fp.mult = mult;
         fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
         fp.factr = 1e+7; fp.pgtol = 1e-5;
+        fp.onepass = 0; fp.a0sum = 0.0;
         double xs[2] = {1.0, 0.0}, f;
         int nev;
         fsx_lbfgsb_fit(&t, &fp, xs, &f, &nev);

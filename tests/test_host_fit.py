@@ -67,3 +67,27 @@ def test_host_fit_stress_bit_identical(F, orc):
             assert (got[0], got[1], got[2], got[3]) == (sc, c1, c2, ne), (c1b, c2b, i)
             nev.append(ne)
     assert max(nev) > 20 and np.mean(nev) > 4
+
+
+@pytest.mark.parametrize("bounds", [((0.96, 1.04), (-2.0, 4.0)), ((0.95, 1.05), (0.0, 4.0))])
+def test_one_pass_fit_matches_oracle(F, orc, bounds):
+    """The mixed-precision form's fit evaluates scale, objective and gradient
+    in one pass over q (six k-independent sums, combined after the pass).
+    The value and gradient equal the reference's two-pass ones up to
+    rounding, so on the same profiles the fits must end where the oracle's
+    do: score within 1e-9 relative, c1 and c2 within 1e-8, and the same
+    ranking of the poses."""
+    pb = build_problem(300, 200, 8, 0.3, 25, np.arange(15.0, 35.0), 300, seed=23)
+    P = O.Params(pb["a"], pb["rm"], pb["peak"], c1=bounds[0], c2=bounds[1])
+    r = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                              P, pb["qvals"], pb["zvals"], pb["L"], want_cross=True)
+    got = []
+    for ct in r["cross"]:
+        sc, c1, c2, nev, cts = O.fit_one(ct, pb["qvals"], P)
+        g = F.fit_host(cts, pb["qvals"], pb["a"], P.s.mult, c1=bounds[0], c2=bounds[1],
+                       mixed=True)
+        assert abs(g[0] - sc) <= 1e-9 * sc
+        assert abs(g[1] - c1) <= 1e-8 and abs(g[2] - c2) <= 1e-8
+        got.append(g[0])
+    assert np.array_equal(np.argsort(r["score"], kind="stable"),
+                          np.argsort(np.array(got), kind="stable"))
