@@ -731,10 +731,15 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             const F *Tq = T + ((long)z * Q + q) * Tn;
             /* pose loader: receptor row entries, twiddles, row; pb >= 0: the
              * record of pose p is sP[p - pb], else it is read from recs */
-            E rr[3][J];
-            F wr[J], wi[J];
-            int prow = 0;
-            auto load_pose = [&](int p, int pb) {
+            /* the current pose's receptor row entries, twiddles and row (a
+             * second register set for the next pose's loads, in flight during
+             * this pose's pairing, measured slower: the kernel then holds
+             * three blocks per CU instead of four) */
+            E rrc[3][J];
+            F wrc[J], wic[J];
+            int prowc = 0;
+            auto load_pose = [&](int p, int pb, E (&rr)[3][J], F (&wr)[J], F (&wi)[J],
+                                 int &prow) {
                 const int s = start + p;
                 prow = 0;
 #pragma unroll
@@ -766,7 +771,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             };
             /* one pose per wavefront: its first receptor row is in flight
              * while U is formed (two poses: loaded after U, registers) */
-            if (PW == 1 && wave < cnt) load_pose(wave, -1);
+            if (PW == 1 && wave < cnt) load_pose(wave, -1, rrc, wrc, wic, prowc);
             if (tid < cnt && tid < FSX_CROSS_PB) {
                 const int4 r4 = recs[start + tid];
                 sP[tid] = make_int2(r4.y, r4.z);
@@ -883,14 +888,11 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 __syncthreads();
             }
             const int pe = min(cnt, pb + FSX_CROSS_PB);
-            for (int pp = wave; pb + PW * pp < pe; pp += nwave) {
-                /* this lane's pose: PW consecutive poses per wavefront (a half
-                 * past the batch's last pose repeats it and does not write) */
-                const int p = pb + PW * pp + half;
+            /* pairing of one pose (this lane's half of the wavefront) with
+             * the pose record's receptor row and twiddles in rr, wr, wi */
+            auto pair_pose = [&](const E (&rr)[3][J], const F (&wr)[J], const F (&wi)[J],
+                                 int prow, int p) {
                 const bool act = p < pe;
-                if (PW > 1 || pb > 0 || pp != wave) load_pose(min(p, pe - 1), pb);
-                /* every load of the pose is issued before the first use */
-                __builtin_amdgcn_sched_barrier(0);
                 const E *sUp = sU + prow * 3 * nc;
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -958,6 +960,17 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 }
                 if (writer && act) sO[(p - pb) * 6 + slot] = v[0];
+            };
+            /* the wavefront's poses of this batch, pp = wave, wave + nwave,
+             * ...: PW consecutive poses per step (a half past the batch's
+             * last pose repeats it and does not write) */
+            for (int pp = wave; pb + PW * pp < pe; pp += nwave) {
+                const int p = pb + PW * pp + half;
+                if (PW > 1 || pb > 0 || pp != wave)
+                    load_pose(min(p, pe - 1), pb, rrc, wrc, wic, prowc);
+                /* every load of the pose is issued before the first use */
+                __builtin_amdgcn_sched_barrier(0);
+                pair_pose(rrc, wrc, wic, prowc, p);
             }
             /* the batch's sums to acc (pose-major [pose][q][6]), added to
              * those of the earlier m-chunks */
