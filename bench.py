@@ -352,7 +352,8 @@ def main():
         fb = n * (6 * pb.qnum * 8 + 24)
         achieved = fb / (avg["ms_fit"] * 1e-3) / 1e9
         roof_fit = {"kernel": "k_fit", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                    "unit": "GB/s", "frac": achieved / 8000.0, "traffic": traffic.get("k_fit"),
+                    "unit": "GB/s", "frac": achieved / 8000.0,
+                    "traffic": traffic.get("k_fit_" + ("f32" if mixed else "f64")),
                     "ms_per_launch": avg["ms_fit"], "algorithmic_bytes": fb,
                     "evaluations_per_fit": float(np.mean([s["fit_evals"] for s in stats]))
                     / max(1, stats[-1]["n_scored"]),
