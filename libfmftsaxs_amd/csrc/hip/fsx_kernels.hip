@@ -604,14 +604,26 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
  *   2 Re sum_e e^{-i m a2} (R^X_e U^Y_e [+ R^Y_e U^X_e])
  * over the chunk's entries e = (l, m), stored (first chunk) or added into
  * acc[(s*Q + q)*6 + k]. */
+/* Workgroup barrier that orders LDS accesses only: unlike __syncthreads it
+ * does not wait for the block's outstanding global stores (the cross-term
+ * accumulator writes, which no other thread of the launch reads), so they
+ * drain while the block goes on. */
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 #define FSX_CROSS_BLOCK 256
 #define FSX_CROSS_PB 64   /* poses per batch of the cross-term pose loop */
-/* FP32 tables: at most 96 VGPRs with two gamma2 rows per tile, so that five
- * blocks of four wavefronts fit a CU; with four rows 128 VGPRs and four
- * blocks (five blocks of 96 VGPRs are slower: 8.1 against 5.9 ms at L = 15);
+/* s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima) */
+#define FSX_WAITCNT_VM0 0x0F70
+/* FP32 tables: at most 128 VGPRs, four blocks of four wavefronts per CU
+ * (five blocks of 96 VGPRs were slower: 8.1 against 5.9 ms at L = 15);
  * FP64 tables: the LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E, int PW>
-__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? (G >= 4 ? 4 : 5) : 1)
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 4 : 1)
 k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
         const int *ent, int cstart, int nc, const int *counts, const int *offs,
         const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
@@ -717,40 +729,93 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
         const E *Bq = Bc + (((long)q * nrows + b2 * ng + g20) * 3) * nc;
         const E *Rq = Rc + (long)q * nrows * 3 * nc;   /* receptor rows of this q */
         bool staged = false;
-        for (int z = 0; z < zn; z++) {
-            /* the tile's buckets are adjacent keys, their poses adjacent in recs */
-            const int key = (z * nbeta + b2) * ng + g20;
-            /* first pose of each row of the tile (rows past nr: none) */
-            int cum[G + 1];
-            cum[0] = 0;
+        /* Each z's bucket header, translation block (TST) and first pose
+         * records are read one z ahead: the next z's header is requested at
+         * the top of an iteration, its translation block before the U step
+         * and its records after it, so that they arrive while this z's U
+         * step and pose loop run instead of stalling the block between z
+         * values.  Header: the tile's buckets are adjacent keys, their poses
+         * adjacent in recs; lane r < nr of every wavefront reads the count
+         * of row r and lane G the first record's offset (vector loads, so
+         * that waiting for them does not wait for the LDS traffic of the U
+         * step), read back by lane broadcast. */
+        auto hload = [&](int zz) {
+            const int key = (zz * nbeta + b2) * ng + g20;
+            int v = 0;
+            if (lane < nr) v = counts[key + lane];
+            else if (lane == G) v = offs[key];
+            return v;
+        };
+        /* first pose of each row of the tile (rows past nr: none) */
+        auto hread = [&](int v, int (&cm)[G + 1], int &st) {
+            cm[0] = 0;
 #pragma unroll
-            for (int r = 0; r < G; r++) cum[r + 1] = cum[r] + (r < nr ? counts[key + r] : 0);
-            const int cnt = cum[G];
-            if (cnt == 0) continue;
-            const int start = offs[key];
+            for (int r = 0; r < G; r++)
+                cm[r + 1] = cm[r] + (r < nr ? __builtin_amdgcn_readlane(v, r) : 0);
+            st = __builtin_amdgcn_readlane(v, G);
+        };
+        /* a translation block straight into sT (TST): each wavefront copies
+         * 64 consecutive 4-byte words per instruction with global_load_lds,
+         * so the copy runs in the background without registers; it is
+         * waited for (vmcnt) before the barrier that precedes the U step */
+        auto tdma = [&](int zz) {
+            const unsigned *Tz = reinterpret_cast<const unsigned *>(T + ((long)zz * Q + q) * Tn);
+            const int nw = Tn * (int)(sizeof(F) / 4);
+            unsigned *dst = reinterpret_cast<unsigned *>(sT);
+            for (int b = wave * FSX_WAVE; b < nw; b += nwave * FSX_WAVE)
+                if (b + lane < nw)
+                    __builtin_amdgcn_global_load_lds(
+                        Tz + b + lane, (__attribute__((address_space(3))) void *)(dst + b), 4, 0, 0);
+        };
+        int2 rpf = make_int2(0, 0);   /* a first record (tid < FSX_CROSS_PB) */
+        auto rload = [&](int st, int n) {
+            if (tid < n && tid < FSX_CROSS_PB) {
+                const int4 r4 = recs[st + tid];
+                rpf = make_int2(r4.y, r4.z);
+            }
+        };
+        int cum[G + 1], start = 0;
+        hread(hload(0), cum, start);
+        int cnt = cum[G];
+        if (TST && cnt > 0) tdma(0);
+        rload(start, cnt);
+        for (int z = 0; z < zn; z++) {
+            const bool zmore = z + 1 < zn;
+            const int hv = zmore ? hload(z + 1) : 0;
+            int cum2[G + 1], start2 = 0;
+#pragma unroll
+            for (int r = 0; r <= G; r++) cum2[r] = 0;
+            if (cnt == 0) {
+                /* an empty z: straight on to the next one */
+                if (zmore) {
+                    hread(hv, cum2, start2);
+                    const int cnt2 = cum2[G];
+                    if (TST && cnt2 > 0) tdma(z + 1);
+                    rload(start2, cnt2);
+#pragma unroll
+                    for (int r = 0; r <= G; r++) cum[r] = cum2[r];
+                    start = start2;
+                    cnt = cnt2;
+                }
+                continue;
+            }
             const F *Tq = T + ((long)z * Q + q) * Tn;
-            /* pose loader: receptor row entries, twiddles, row; pb >= 0: the
-             * record of pose p is sP[p - pb], else it is read from recs */
-            /* the current pose's receptor row entries, twiddles and row (a
-             * second register set for the next pose's loads, in flight during
-             * this pose's pairing, measured slower: the kernel then holds
-             * three blocks per CU instead of four) */
+            /* pose loader: receptor row entries, twiddles and row of pose p,
+             * whose record is sP[p - pb].  One register set (a second one for
+             * the next pose's loads, in flight during this pose's pairing,
+             * measured slower: the kernel then holds three blocks per CU
+             * instead of four); the first pose is loaded after the U step
+             * (loading it before kept its registers live through U and cost
+             * FP64 tables a wavefront per SIMD) */
             E rrc[3][J];
             F wrc[J], wic[J];
             int prowc = 0;
             auto load_pose = [&](int p, int pb, E (&rr)[3][J], F (&wr)[J], F (&wi)[J],
                                  int &prow) {
-                const int s = start + p;
                 prow = 0;
 #pragma unroll
                 for (int r = 1; r < G; r++) prow += p >= cum[r];
-                int2 rec;
-                if (pb >= 0) {
-                    rec = sP[p - pb];
-                } else {
-                    const int4 r4 = recs[s];
-                    rec = make_int2(r4.y, r4.z);
-                }
+                const int2 rec = sP[p - pb];
                 /* no branches, so that the J slots' loads are all in flight
                  * together: a lane's slot past the last one (only the last
                  * of the J can be) reads slot 0, and the pairing gives it a
@@ -769,13 +834,12 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     for (int X = 0; X < 3; X++) rr[X][j] = Rj[X * nc];
                 }
             };
-            /* one pose per wavefront: its first receptor row is in flight
-             * while U is formed (two poses: loaded after U, registers) */
-            if (PW == 1 && wave < cnt) load_pose(wave, -1, rrc, wrc, wic, prowc);
-            if (tid < cnt && tid < FSX_CROSS_PB) {
-                const int4 r4 = recs[start + tid];
-                sP[tid] = make_int2(r4.y, r4.z);
-            }
+            /* the translation block copied in the background (TST) has
+             * landed in sT (the barrier before the U step makes it visible
+             * to the other wavefronts) */
+            if (TST) __builtin_amdgcn_s_waitcnt(FSX_WAITCNT_VM0);
+
+            if (tid < cnt && tid < FSX_CROSS_PB) sP[tid] = rpf;
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
@@ -800,20 +864,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 }
                 staged = true;
             }
-            if (TST) {
-                for (int b0 = tid; b0 < Tn; b0 += 4 * FSX_CROSS_BLOCK) {
-                    const int i1 = b0 + FSX_CROSS_BLOCK, i2 = b0 + 2 * FSX_CROSS_BLOCK;
-                    const int i3 = b0 + 3 * FSX_CROSS_BLOCK;
-                    const int j1 = i1 < Tn ? i1 : b0, j2 = i2 < Tn ? i2 : b0;
-                    const int j3 = i3 < Tn ? i3 : b0;
-                    F t0 = Tq[b0], t1 = Tq[j1], t2 = Tq[j2], t3 = Tq[j3];
-                    sT[b0] = t0;
-                    if (i1 < Tn) sT[i1] = t1;
-                    if (i2 < Tn) sT[i2] = t2;
-                    if (i3 < Tn) sT[i3] = t3;
-                }
-            }
-            __syncthreads();
+            lds_barrier();
             {
                 /* T = i^(l1 - l) tau with tau real and the ligand rows stored
                  * as i^(-l1) B, so that i^(-l) conj(T) B = tau (i^(-l1) B): a
@@ -876,7 +927,16 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                         }
                 }
             }
-            __syncthreads();
+            lds_barrier();
+            /* the next z's header, and its first records and translation
+             * block in flight during this z's pose loop (U has read sT) */
+            int cnt2 = 0;
+            if (zmore) {
+                hread(hv, cum2, start2);
+                cnt2 = cum2[G];
+                rload(start2, cnt2);
+                if (TST && cnt2 > 0) tdma(z + 1);
+            }
             /* batches of FSX_CROSS_PB poses; the records of the first are
              * staged with the ligand rows */
             for (int pb = 0; pb < cnt; pb += FSX_CROSS_PB) {
@@ -885,7 +945,7 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                     const int4 r4 = recs[start + pb + tid];
                     sP[tid] = make_int2(r4.y, r4.z);
                 }
-                __syncthreads();
+                lds_barrier();
             }
             const int pe = min(cnt, pb + FSX_CROSS_PB);
             /* pairing of one pose (this lane's half of the wavefront) with
@@ -966,15 +1026,14 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
              * last pose repeats it and does not write) */
             for (int pp = wave; pb + PW * pp < pe; pp += nwave) {
                 const int p = pb + PW * pp + half;
-                if (PW > 1 || pb > 0 || pp != wave)
-                    load_pose(min(p, pe - 1), pb, rrc, wrc, wic, prowc);
+                load_pose(min(p, pe - 1), pb, rrc, wrc, wic, prowc);
                 /* every load of the pose is issued before the first use */
                 __builtin_amdgcn_sched_barrier(0);
                 pair_pose(rrc, wrc, wic, prowc, p);
             }
             /* the batch's sums to acc (pose-major [pose][q][6]), added to
              * those of the earlier m-chunks */
-            __syncthreads();
+            lds_barrier();
             for (int i = tid; i < (pe - pb) * 6; i += FSX_CROSS_BLOCK) {
                 const int ip = i / 6;
                 double *d = acc + ((long)(start + pb + ip) * Q + q) * 6 + (i - ip * 6);
@@ -982,7 +1041,11 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
                 *d = first ? x : *d + x;
             }
             }
-            __syncthreads();
+            lds_barrier();
+#pragma unroll
+            for (int r = 0; r <= G; r++) cum[r] = cum2[r];
+            start = start2;
+            cnt = cnt2;
         }
     }
 }
