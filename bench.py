@@ -343,12 +343,24 @@ def main():
         # priced against HBM (its profile read once, three results written).
         peak = 78.6
         achieved = cf / (avg["ms_cross"] * 1e-3) / 1e12
+        tr = traffic.get("k_cross_" + ("f32" if mixed else "f64"))
         roof_cross = {"kernel": "k_cross", "bound": "valu", "achieved": achieved, "peak": peak,
                       "peak_basis": "VALU FMA rate, FP64 and non-packed FP32",
-                      "unit": "TFLOP/s", "frac": achieved / peak,
-                      "traffic": traffic.get("k_cross_" + ("f32" if mixed else "f64")),
+                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": tr,
                       "ms_per_launch": avg["ms_cross"], "algorithmic_flops": cf,
                       "algorithmic_bytes": cb, "arith": "f32" if mixed else "f64"}
+        if tr:
+            # the measured HBM bytes per launch as a rate: when it is closer to
+            # the HBM peak than the flops are to the VALU peak (receptor rows
+            # that no longer fit L2, e.g. L = 25), HBM is the binding roof and
+            # the kernel is priced against it with its algorithmic bytes
+            rate = tr / (avg["ms_cross"] * 1e-3) / 1e9
+            roof_cross["traffic_gbs"] = rate
+            if rate / 8000.0 > roof_cross["frac"]:
+                ab = cb / (avg["ms_cross"] * 1e-3) / 1e9
+                roof_cross.update({"bound": "hbm", "achieved": ab, "peak": 8000.0,
+                                   "peak_basis": "HBM3E", "unit": "GB/s", "frac": ab / 8000.0,
+                                   "valu_tflops": achieved, "valu_frac": achieved / peak})
         fb = n * (6 * pb.qnum * 8 + 24)
         achieved = fb / (avg["ms_fit"] * 1e-3) / 1e9
         roof_fit = {"kernel": "k_fit", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
