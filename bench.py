@@ -350,14 +350,19 @@ def main():
                       "ms_per_launch": avg["ms_cross"], "algorithmic_flops": cf,
                       "algorithmic_bytes": cb, "arith": "f32" if mixed else "f64"}
         if tr:
-            # the measured HBM bytes per launch as a rate: when it is closer to
-            # the HBM peak than the flops are to the VALU peak (receptor rows
-            # that no longer fit L2, e.g. L = 25), HBM is the binding roof and
-            # the kernel is priced against it with its algorithmic bytes
+            # the measured HBM bytes per launch as a rate.  The algorithmic
+            # bytes (one receptor row per pose and q) overstate what must come
+            # from HBM wherever L2 serves repeated rows (L = 15: 40.6 GB asked,
+            # 15.7 GB fetched), so HBM is taken as the binding roof only when
+            # the measured rate is clearly nearer its peak than the flops are
+            # to theirs (receptor rows that no longer fit L2, e.g. L = 25), and
+            # then priced with the smaller of the two byte counts: bytes the
+            # caches served never count as HBM throughput
             rate = tr / (avg["ms_cross"] * 1e-3) / 1e9
             roof_cross["traffic_gbs"] = rate
-            if rate / 8000.0 > roof_cross["frac"]:
-                ab = cb / (avg["ms_cross"] * 1e-3) / 1e9
+            roof_cross["traffic_frac"] = rate / 8000.0
+            if rate / 8000.0 > 1.25 * roof_cross["frac"]:
+                ab = min(cb, tr) / (avg["ms_cross"] * 1e-3) / 1e9
                 roof_cross.update({"bound": "hbm", "achieved": ab, "peak": 8000.0,
                                    "peak_basis": "HBM3E", "unit": "GB/s", "frac": ab / 8000.0,
                                    "valu_tflops": achieved, "valu_frac": achieved / peak})

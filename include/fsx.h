@@ -47,7 +47,11 @@ extern "C" {
 #define FSX_MIXED_FP32 8u      /* mixed-precision form: rotated coefficients, translation
                                   blocks and the cross-term contraction in FP32 (Bessel,
                                   Wigner-d, the lane reductions, the per-pose accumulators
-                                  and the chi^2 fit stay FP64) */
+                                  and the chi^2 fit stay FP64); the fit is L-BFGS-B with
+                                  the 2 x 2 L-BFGS matrix held explicitly, run by lane
+                                  groups that hold the profile in registers */
+#define FSX_FIT_COMPACT 16u    /* with FSX_MIXED_FP32: fit with the compact-form L-BFGS-B
+                                  (the reference's factorisations, lane per fit) instead */
 
 /* pi of the reference CMake build (-std=c11 leaves M_PI undefined, so
  * define.h:13-15 supplies this truncated value) */
@@ -127,8 +131,11 @@ int fsx_fit_host(const double *ct, const double *qvals, int qnum, const double *
 
 /* fsx_fit_host in either form: flags 0 (the reference's evaluation order,
  * iterates bit-identical to sxs_lbfgs_fitting) or FSX_MIXED_FP32 (the
- * mixed-precision form's fit: scale, objective and gradient from one pass
- * over q, equal to the two-pass values up to rounding).  No counterpart in
+ * mixed-precision form's fit: value and gradient from one pass over q, equal
+ * to the two-pass ones up to rounding, and L-BFGS-B with the 2 x 2 L-BFGS
+ * matrix held explicitly, whose steps are the reference's up to rounding;
+ * with FSX_FIT_COMPACT as well, the reference's compact-form L-BFGS-B on the
+ * one-pass values).  out4[3] is the number of evaluations.  No counterpart in
  * the reference; it lets a caller reproduce either form's fit on the host. */
 int fsx_fit_host_form(const double *ct, const double *qvals, int qnum, const double *a,
                       double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,

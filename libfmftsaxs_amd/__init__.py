@@ -17,6 +17,7 @@ PI_REFERENCE = 3.14159265358
 FSX_DEVICE_POSES = 1
 FSX_DEVICE_OUTPUT = 2
 FSX_MIXED_FP32 = 8
+FSX_FIT_COMPACT = 16
 
 
 class FsxError(RuntimeError):
@@ -190,10 +191,11 @@ class ScoringProblem:
         return self.s.qnum
 
 
-def score(problem, poses, want_cross=False, out=None, mixed=False):
+def score(problem, poses, want_cross=False, out=None, mixed=False, fit_compact=False):
     """Score poses (int64 mixed-radix indices).  Returns dict with score, c1,
     c2 (NaN where the z slot is out of range), optional cross [n,6,Q], stats.
-    mixed=True selects the mixed-precision form (FP32 contraction)."""
+    mixed=True selects the mixed-precision form (FP32 contraction, lane-group
+    fit); fit_compact=True gives that form the compact-form L-BFGS-B fit."""
     poses = np.ascontiguousarray(poses, dtype=np.int64)
     n = poses.size
     sc = np.full(n, np.nan)
@@ -204,7 +206,8 @@ def score(problem, poses, want_cross=False, out=None, mixed=False):
     _chk(lib().fsx_score(C.byref(problem.s), poses.ctypes.data, n, sc.ctypes.data,
                          c1.ctypes.data, c2.ctypes.data,
                          None if cross is None else cross.ctypes.data,
-                         FSX_MIXED_FP32 if mixed else 0, C.byref(st)))
+                         (FSX_MIXED_FP32 if mixed else 0) | (FSX_FIT_COMPACT if fit_compact else 0),
+                         C.byref(st)))
     return {"score": sc, "c1": c1, "c2": c2, "cross": cross, "stats": st.as_dict()}
 
 
@@ -246,22 +249,27 @@ def score_batch_device(problems, d_poses, ns, d_score, d_c1, d_c2, mixed=False):
     return _batch_call(problems, d_poses, ns, d_score, d_c1, d_c2, None, flags)
 
 
-def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0, mixed=False):
+def score_device(problem, d_poses, n, d_score, d_c1, d_c2, d_cross=0, mixed=False,
+                 fit_compact=False):
     """Device-resident form: all pointers are device addresses (ints)."""
     st = Stats()
-    flags = FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT | (FSX_MIXED_FP32 if mixed else 0)
+    flags = (FSX_DEVICE_POSES | FSX_DEVICE_OUTPUT | (FSX_MIXED_FP32 if mixed else 0)
+             | (FSX_FIT_COMPACT if fit_compact else 0))
     _chk(lib().fsx_score(C.byref(problem.s), d_poses, n, d_score, d_c1, d_c2, d_cross or None,
                          flags, C.byref(st)))
     return st.as_dict()
 
 
-def fit_host(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0), mixed=False):
-    """Host fit of one rescaled profile ct [6,Q] -> (score, c1, c2, evals); mixed: the
-    mixed-precision form's one-pass evaluation instead of the reference's two passes."""
+def fit_host(ct, qvals, a, mult, c1=(0.96, 1.04), c2=(-2.0, 4.0), mixed=False,
+             compact=False):
+    """Host fit of one rescaled profile ct [6,Q] -> (score, c1, c2, evals).  mixed: the
+    mixed-precision form's fit (one-pass evaluation, L-BFGS-B with the explicit 2 x 2
+    matrix; with compact, the compact-form L-BFGS-B) instead of the reference's."""
     ct = f64(ct)
     out = np.zeros(4)
+    flags = (FSX_MIXED_FP32 | (FSX_FIT_COMPACT if compact else 0)) if mixed else 0
     _chk(lib().fsx_fit_host_form(ct.reshape(-1), f64(qvals), ct.shape[-1], f64(a), mult,
-                                 c1[0], c1[1], c2[0], c2[1], FSX_MIXED_FP32 if mixed else 0, out))
+                                 c1[0], c1[1], c2[0], c2[1], flags, out))
     return out
 
 

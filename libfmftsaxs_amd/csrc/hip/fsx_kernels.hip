@@ -33,6 +33,7 @@
 #define FSX_LB_KSHARED
 #define FSX_LB_KSTRIDE 256
 #include "../fsx_lbfgsb.h"
+#include "../fsx_lb2.h"
 
 #define FSX_CROSS_THREADS 256
 #define FSX_WAVE 64
@@ -1586,6 +1587,165 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
     }
 }
 
+/* ---------------------------------------------------------------------- */
+/* The mixed-precision form's fit: the two-variable L-BFGS-B of fsx_lb2.h
+ * (the reference's steps with the 2 x 2 L-BFGS matrix held explicitly),
+ * one pose per group of fpl lanes (a power of two).  Lane j of a group
+ * evaluates the q range [j qpl, (j + 1) qpl), qpl = ceil(Q / fpl), reading
+ * the pose's profile at each evaluation (one-pass form, fsx_lbfgsb.h
+ * FSX_1P_TERMS): its six sums run in q order from its second q on, its
+ * first q pairs with the previous lane's last (a lane shift), and the sums
+ * meet by an xor butterfly, after which every lane of the group holds the
+ * same value and gradient and runs the same solver step.  Between steps the
+ * solver state lives in LDS, one copy per group (every lane of the group
+ * writes the same values), so that the kernel fits in 168 registers: three
+ * wavefronts per SIMD, against one for the compact form's state.  With four
+ * lanes per fit (Q up to 64) a wavefront runs 16 fits, so the solver's
+ * divergent code is shared by many fits while the evaluation of each is
+ * split four ways (on the 210,000-pose L = 15 workload: 1.64 ms, against
+ * 1.83 ms at two wavefronts per SIMD and 1.87 ms with eight lanes).  The moments, q and the step reciprocals sit in LDS, slot
+ * ii * fpl + j for q = j qpl + ii, so the lanes of a group read consecutive
+ * doubles.  Groups take poses from a queue (next) until none is left. */
+#define FSX_FG_ARR 8     /* LDS arrays: a0..a5, q, 1 / (q_i - q_{i-1}) */
+#define FSX_FG_QLANE 16  /* q per lane the default lanes per fit aim at */
+#ifndef FSX_FG_WAVES
+#define FSX_FG_WAVES 3   /* wavefronts per SIMD the register budget allows */
+#endif
+
+__device__ __forceinline__ void fitg_eval(const double *pr, const double *sNt, int ns, int qpl,
+                                          int fpl, int j, int Q, double mult, double a0sum,
+                                          const double *x, double *fv, double *g)
+{
+#pragma clang fp contract(fast)
+    const double c1 = x[0], c2 = x[1];
+    const double corr = -mult * (c1 * c1 - 1.0), c1c = c1 * c1 * c1;
+    const int q0 = j * qpl, nq = Q - q0 < qpl ? Q - q0 : qpl;
+    double s[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = 0.0;
+    double vf[3] = {0.0, 0.0, 0.0}, vp[3] = {0.0, 0.0, 0.0};
+    /* the sums' terms of step ii from I and its derivatives there (vv) and
+     * at the previous q (pp) */
+    auto terms_at = [&](const double *vv, const double *pp, int ii) {
+        const int slot = ii * fpl + j;
+        double ai[6], tt[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) ai[k] = sNt[k * ns + slot];
+        FSX_1P_TERMS(vv[0], vv[1], vv[2], pp[0], pp[1], pp[2], sNt[7 * ns + slot],
+                     sNt[6 * ns + slot], ai, tt);
+#pragma unroll
+        for (int k = 0; k < 6; k++) s[k] += tt[k];
+    };
+    for (int ii = 0; ii < nq; ii++) {
+        const double2 *p2 = reinterpret_cast<const double2 *>(pr + 6 * (q0 + ii));
+        const double2 a = p2[0], b = p2[1], c = p2[2];
+        const double cur[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+        /* I and its derivatives in c1, c2 (min_saxs.c:52-68) */
+        const double qc = sNt[6 * ns + ii * fpl + j];
+        const double G = c1c * exp(corr * qc * qc);
+        const double Gd = G * (3.0 / c1 - 2.0 * c1 * mult * qc * qc);
+        double vv[3];
+        FSX_OBJ_VALS(cur, G, Gd, c2, vv[0], vv[1], vv[2]);
+        if (ii == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) vf[k] = vv[k];
+        } else {
+            terms_at(vv, vp, ii);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) vp[k] = vv[k];
+    }
+    /* the first q of a lane pairs with the previous lane's last (lane 0:
+     * q = 0 with itself, slope 0, as min_saxs.c:30-35 starts) */
+    double pl[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) pl[k] = __shfl_up(vp[k], 1, fpl);
+    if (nq > 0) terms_at(vf, j == 0 ? vf : pl, 0);
+    for (int o = 1; o < fpl; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) s[k] += __shfl_xor(s[k], o, fpl);
+    }
+    fsx_1p_finish(s, a0sum, fv, g);
+}
+
+__global__ void __launch_bounds__(256, FSX_FG_WAVES)
+k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+       double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
+{
+    extern __shared__ double sNt[];
+    const int qpl = (Q + fpl - 1) / fpl;
+    const int ns = qpl * fpl;
+    fsx_lb2 *sSt = reinterpret_cast<fsx_lb2 *>(sNt + FSX_FG_ARR * ns);
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const int q = (i % fpl) * qpl + i / fpl;
+        const bool ok = q < Q;
+#pragma unroll
+        for (int k = 0; k < 6; k++) sNt[k * ns + i] = ok ? a[6 * q + k] : 0.0;
+        sNt[6 * ns + i] = ok ? qvals[q] : 0.0;
+        /* min_saxs.c:60: 1 / (q_i - q_{i-1}) with q_{-1} = -1 */
+        sNt[7 * ns + i] = ok ? 1.0 / (qvals[q] - (q > 0 ? qvals[q - 1] : -1.0)) : 0.0;
+    }
+    __syncthreads();
+    fsx_fitpar fp;
+    fp.lo0 = c1_lo; fp.hi0 = c1_hi; fp.lo1 = c2_lo; fp.hi1 = c2_hi;
+    fp.factr = 1e+7; fp.pgtol = 1e-5;
+    const double a0sum = fsx_a0sum(a, Q);
+    const int nvalid = *nvalid_p;
+    const int lane = threadIdx.x & (FSX_WAVE - 1), j = lane & (fpl - 1);
+    const int leader = lane & ~(fpl - 1);
+    int s = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / fpl);
+    unsigned long long nev = 0, ncap = 0, nmax = 0;
+    fsx_lb2 *S = sSt + threadIdx.x / fpl;
+    while (s < nvalid) {
+        const double *pr = acc + (long)s * 6 * Q;
+        fsx_lb2_start(S, &fp, 1.0, 0.0);   /* define.h:33-34 */
+        int more;
+        do {
+            double xt[2] = {S->x[0], S->x[1]}, fv, gv[2];
+            /* the state is read back from LDS after the evaluation rather
+             * than kept in registers across it */
+            asm volatile("" ::: "memory");
+            fitg_eval(pr, sNt, ns, qpl, fpl, j, Q, mult, a0sum, xt, &fv, gv);
+            asm volatile("" ::: "memory");
+            S->f = fv;
+            S->g[0] = gv[0];
+            S->g[1] = gv[1];
+            more = fsx_lb2_resume(S, &fp);
+        } while (more);
+        if (j == 0) {
+            const int o = recs ? recs[s].x : s;
+            score[o] = sqrt(S->f);
+            c1o[o] = S->x[0];
+            c2o[o] = S->x[1];
+        }
+        nev += S->nev;
+        ncap += S->task == FSX_T_STOP_ITER;
+        nmax = nmax > (unsigned long long)S->nev ? nmax : (unsigned long long)S->nev;
+        int sn = 0;
+        if (j == 0) sn = atomicAdd(next, 1);
+        s = __shfl(sn, leader, FSX_WAVE);
+    }
+    if (j == 0) {
+        /* evaluations; fits stopped at the evaluation cap; evaluations again
+         * as the loop count of the lane groups; none wavefront-wide; the
+         * longest fit's evaluations */
+        if (nev) {
+            atomicAdd(&tally[0], nev);
+            atomicAdd(&tally[2], nev);
+        }
+        if (ncap) atomicAdd(&tally[1], ncap);
+        if (nmax) atomicMax(&tally[4], nmax);
+    }
+}
+
+/* LDS of a k_fitg block: the moment arrays and one solver state per group */
+static inline size_t fitg_lds(int Q, int fpl)
+{
+    const size_t qpl = ((size_t)Q + fpl - 1) / fpl;
+    return (size_t)FSX_FG_ARR * qpl * fpl * sizeof(double) + (size_t)(256 / fpl) * sizeof(fsx_lb2);
+}
+
 /* fits of caller-supplied (rescaled) profiles ct[n][6][Q] */
 __global__ void __launch_bounds__(256)
 k_fit_batch(const double *ct, int n, const double *qvals, int Q, const double *a, double mult,
@@ -2138,7 +2298,9 @@ struct fsx_plan {
     long lanes;
     fit_kernel_t kfit;
     int fit_wide;        /* lanes with a fit at or below which a fit wavefront
-                            evaluates them together */
+                            evaluates them together (L-BFGS-B fits); lanes per
+                            fit (Newton fits) */
+    long fit_next0;      /* first queue entry not taken at launch */
     int tile_rows;       /* gamma2 rows per cross-term tile (0: the form's default) */
     int rot_gt;          /* gamma per block of the MFMA rotation (16 or 8), 0: the
                             rotation-then-transpose kernels */
@@ -2297,10 +2459,36 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
      * state lives in registers, one 256-thread block per CU) */
     p->aq_lds = fit_aq_lds(Q);
     p->lds_fit = fit_lds(Q);
-    /* the mixed-precision form's fit evaluates in one pass over q */
+    /* the mixed-precision form's fit: the two-variable L-BFGS-B on lane
+     * groups (k_fitg), or with FSX_FIT_COMPACT (or a q grid whose arrays do
+     * not fit the block's LDS) the compact-form L-BFGS-B lane per fit, both
+     * evaluated in one pass over q; the FP64 form: the reference's
+     * L-BFGS-B, lane per fit.  Lanes per fit of the group fit: about
+     * FSX_FG_QLANE q per lane, at least two (FSX_FIT_GROUP_LANES
+     * overrides), doubled until the block's LDS fits */
+    int fpl = 2;
+    while (fpl < FSX_WAVE && fpl * FSX_FG_QLANE < Q) fpl *= 2;
+    fpl = env_int("FSX_FIT_GROUP_LANES", fpl, 1, FSX_WAVE);
+    while (fpl & (fpl - 1)) fpl &= fpl - 1;   /* a power of two */
+    while (fpl < FSX_WAVE && fitg_lds(Q, fpl) > FSX_FIT_LDS) fpl *= 2;
+    const int grp = p->mixed && !(A->flags & FSX_FIT_COMPACT) && fitg_lds(Q, fpl) <= FSX_FIT_LDS;
     p->kfit = p->aq_lds ? (p->mixed ? k_fit<1, 1> : k_fit<1, 0>)
                         : (p->mixed ? k_fit<0, 1> : k_fit<0, 0>);
     p->fit_wide = env_int("FSX_FIT_WIDE", 24, 0, 64);
+    if (grp) {
+        p->kfit = k_fitg;
+        p->fit_wide = fpl;
+        p->lds_fit = fitg_lds(Q, fpl);
+        int bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
+        if (bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, p->kfit, 256,
+                                                                      p->lds_fit) != hipSuccess ||
+                         bpc < 1))
+            bpc = 1;
+        p->lanes = (long)g_ncu * 256 * bpc;
+        if (p->lanes > (long)n * fpl) p->lanes = ((long)n * fpl + 255) / 256 * 256;
+        p->fit_next0 = p->lanes / fpl;
+        return FSX_OK;
+    }
     int fit_bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
     if (fit_bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit_bpc, p->kfit, 256,
                                                                       p->lds_fit) != hipSuccess ||
@@ -2310,6 +2498,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     const int fit_blocks = env_int("FSX_FIT_BLOCKS", 0, 0, 1 << 20);
     if (fit_blocks > 0) p->lanes = 256L * fit_blocks;
     if (p->lanes > n) p->lanes = ((long)n + 255) / 256 * 256;
+    p->fit_next0 = p->lanes;
     return FSX_OK;
 }
 
@@ -2486,7 +2675,7 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[4], s));
     if (stop_after_cross) return FSX_OK;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_NEXT], 1L, (int)p->lanes);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_NEXT], 1L, (int)p->fit_next0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_TALLY], 2L * FSX_TALLY_N, 0);
     /* rescaled profiles in place of the pose parts (bounded by n slots;
      * the kernels read the number of valid slots on the device) */

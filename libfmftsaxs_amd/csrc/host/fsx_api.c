@@ -7,6 +7,7 @@
  */
 #include "../fsx_internal.h"
 #include "../fsx_lbfgsb.h"
+#include "../fsx_lb2.h"
 #include "fftsaxs.h"
 #include "min_saxs.h"
 #include "pdb2spf.h"
@@ -478,7 +479,10 @@ int fsx_fit_host_form(const double *ct, const double *qvals, int qnum, const dou
     fp.a0sum = fsx_a0sum(a, qnum);
     double x[2] = {1.0, 0.0}, f;
     int nev;
-    fsx_lbfgsb_fit(&t, &fp, x, &f, &nev);
+    if ((flags & FSX_MIXED_FP32) && !(flags & FSX_FIT_COMPACT))
+        fsx_lb2_fit(&t, &fp, x, &f, &nev);
+    else
+        fsx_lbfgsb_fit(&t, &fp, x, &f, &nev);
     out4[0] = sqrt(f);
     out4[1] = x[0];
     out4[2] = x[1];

@@ -7,8 +7,8 @@ every sum added in q order by one lane.  The fits therefore visit the same
 points and end with the same bits whether the wide form is never used (0),
 always used (64) or used in the tail only (the default).  Checked for one
 round of q (Q <= 64), several rounds with the moments in LDS (Q = 150) and
-from global memory (Q = 400), in both forms; the evaluation counts must
-agree as well.
+from global memory (Q = 400), in both forms (the mixed form with its
+compact-form fit, FSX_FIT_COMPACT); the evaluation counts must agree as well.
 """
 import numpy as np
 import pytest
@@ -45,7 +45,7 @@ def test_wide_evaluation_bit_identical(F, monkeypatch, L, Q, npose, seed, mixed)
         else:
             monkeypatch.setenv("FSX_FIT_WIDE", wide)
         print(f"L={L} Q={Q} mixed={mixed} FSX_FIT_WIDE={wide}", flush=True)
-        res[wide] = F.score(gp, pb["index"], mixed=mixed)
+        res[wide] = F.score(gp, pb["index"], mixed=mixed, fit_compact=mixed)
     base = res["0"]
     ok = ~np.isnan(base["score"])
     assert ok.sum() > 0.9 * npose
