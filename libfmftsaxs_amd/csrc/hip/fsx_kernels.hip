@@ -643,11 +643,11 @@ __device__ __forceinline__ void lds_barrier()
  * (five blocks of 96 VGPRs were slower: 8.1 against 5.9 ms at L = 15);
  * FP64 tables: the LDS of three blocks fills a CU, no register bound */
 template <int J, int TST, int G, typename E, int PW>
-__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 4 : 1)
-k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
-        const int *ent, int cstart, int nc, const int *counts, const int *offs,
-        const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
-        int zn, int first, int npk)
+__device__ __forceinline__ void
+cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
+           const int *ent, int cstart, int nc, const int *counts, const int *offs,
+           const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
+           int zn, int first, int npk)
 {
 #pragma clang fp contract(fast)
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
@@ -1067,6 +1067,48 @@ k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E
             start = start2;
             cnt = cnt2;
         }
+    }
+}
+
+template <int J, int TST, int G, typename E, int PW>
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 4 : 1)
+k_cross(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, const E *tw2,
+        const int *ent, int cstart, int nc, const int *counts, const int *offs,
+        const int4 *recs, long ntiles, int nrows, int Q, int L, int nbeta, int nx, int ng,
+        int zn, int first, int npk)
+{
+    cross_body<J, TST, G, E, PW>(acc, Rc, Bc, T, tw2, ent, cstart, nc, counts, offs, recs, ntiles,
+                                 nrows, Q, L, nbeta, nx, ng, zn, first, npk);
+}
+
+/* One m-chunk of several complexes in one persistent launch (the batched
+ * form): every block takes its share of the tiles of job 0, then of job 1,
+ * and so on, so that a block that runs out of one complex's tiles goes on
+ * to the next complex instead of idling until the launch drains.  The jobs
+ * of one launch share the template parameters; each complex appears at most
+ * once (its m-chunks are launched in order, one launch per chunk index). */
+struct fsx_cross_job {
+    double *acc;
+    const void *Rc, *Bc, *T, *tw2;
+    const int *ent, *counts, *offs;
+    const int4 *recs;
+    long ntiles;
+    int cstart, nc, nrows, Q, L, nbeta, nx, ng, zn, first, npk;
+};
+
+template <int J, int TST, int G, typename E, int PW>
+__global__ void __launch_bounds__(FSX_CROSS_BLOCK, sizeof(E) == 8 ? 4 : 1)
+k_cross_jobs(const fsx_cross_job *jobs, int njobs)
+{
+    typedef decltype(E::re) F;
+    for (int k = 0; k < njobs; k++) {
+        const fsx_cross_job &jb = jobs[k];
+        cross_body<J, TST, G, E, PW>(jb.acc, (const E *)jb.Rc, (const E *)jb.Bc, (const F *)jb.T,
+                                     (const E *)jb.tw2, jb.ent, jb.cstart, jb.nc, jb.counts,
+                                     jb.offs, jb.recs, jb.ntiles, jb.nrows, jb.Q, jb.L, jb.nbeta,
+                                     jb.nx, jb.ng, jb.zn, jb.first, jb.npk);
+        /* the next job lays out the LDS for its own chunk size */
+        __syncthreads();
     }
 }
 
@@ -1668,10 +1710,10 @@ __device__ __forceinline__ void fitg_eval(const double *pr, const double *sNt, i
     fsx_1p_finish(s, a0sum, fv, g);
 }
 
-__global__ void __launch_bounds__(256, FSX_FG_WAVES)
-k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
-       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
-       double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
+__device__ __forceinline__ void
+fitg_body(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+          const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+          double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
 {
     extern __shared__ double sNt[];
     const int qpl = (Q + fpl - 1) / fpl;
@@ -1736,6 +1778,43 @@ k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const do
         }
         if (ncap) atomicAdd(&tally[1], ncap);
         if (nmax) atomicMax(&tally[4], nmax);
+    }
+}
+
+__global__ void __launch_bounds__(256, FSX_FG_WAVES)
+k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+       const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
+       double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
+{
+    fitg_body(acc, recs, nvalid_p, Q, qvals, a, mult, c1_lo, c1_hi, c2_lo, c2_hi, score, c1o,
+              c2o, next, tally, fpl);
+}
+
+/* The fits of several complexes in one persistent launch (the batched
+ * form): every block runs the lane groups of job 0 until its pose queue is
+ * empty, then those of job 1, and so on (each job with its own queue, q
+ * grid, moments, bounds and lanes per fit). */
+struct fsx_fit_job {
+    const double *acc, *qvals, *a;
+    const int4 *recs;
+    const int *nvalid_p;
+    double mult, c1_lo, c1_hi, c2_lo, c2_hi;
+    double *score, *c1o, *c2o;
+    int *next;
+    unsigned long long *tally;
+    int Q, fpl;
+};
+
+__global__ void __launch_bounds__(256, FSX_FG_WAVES)
+k_fitg_jobs(const fsx_fit_job *jobs, int njobs)
+{
+    for (int k = 0; k < njobs; k++) {
+        const fsx_fit_job &jb = jobs[k];
+        fitg_body(jb.acc, jb.recs, jb.nvalid_p, jb.Q, jb.qvals, jb.a, jb.mult, jb.c1_lo,
+                  jb.c1_hi, jb.c2_lo, jb.c2_hi, jb.score, jb.c1o, jb.c2o, jb.next, jb.tally,
+                  jb.fpl);
+        /* the next job refills the LDS arrays */
+        __syncthreads();
     }
 }
 
@@ -2175,6 +2254,112 @@ static void launch_cross_g(hipStream_t s, int C, const int *csz, const int *cst,
     }
 }
 
+/* The persistent form of the cross-term stage (fsx_score_batch): each
+ * complex's m-chunks become jobs, grouped into launches of k_cross_jobs by
+ * chunk index and kernel shape, with the shape choices of launch_cross_g. */
+struct fsx_cross_group {
+    int mixed, G, tst, jn, pw, chunk;   /* the kernel shape and the m-chunk index */
+    size_t lds;                         /* the largest job's LDS */
+    int njobs, cap;
+    long grid;                          /* blocks of the launch */
+    fsx_cross_job *jobs;
+};
+
+template <typename E, int G>
+static const void *cross_jobs_kernel(int tst, int jn, int pw)
+{
+    if (pw == 2)
+        return tst ? (jn <= 1 ? (const void *)k_cross_jobs<1, 1, G, E, 2>
+                      : jn <= 2 ? (const void *)k_cross_jobs<2, 1, G, E, 2>
+                      : jn <= 3 ? (const void *)k_cross_jobs<3, 1, G, E, 2>
+                                : (const void *)k_cross_jobs<4, 1, G, E, 2>)
+                   : (jn <= 1 ? (const void *)k_cross_jobs<1, 0, G, E, 2>
+                      : jn <= 2 ? (const void *)k_cross_jobs<2, 0, G, E, 2>
+                      : jn <= 3 ? (const void *)k_cross_jobs<3, 0, G, E, 2>
+                                : (const void *)k_cross_jobs<4, 0, G, E, 2>);
+    return tst ? (jn <= 1 ? (const void *)k_cross_jobs<1, 1, G, E, 1>
+                  : jn <= 2 ? (const void *)k_cross_jobs<2, 1, G, E, 1>
+                  : jn <= 3 ? (const void *)k_cross_jobs<3, 1, G, E, 1>
+                            : (const void *)k_cross_jobs<4, 1, G, E, 1>)
+               : (jn <= 1 ? (const void *)k_cross_jobs<1, 0, G, E, 1>
+                  : jn <= 2 ? (const void *)k_cross_jobs<2, 0, G, E, 1>
+                  : jn <= 3 ? (const void *)k_cross_jobs<3, 0, G, E, 1>
+                            : (const void *)k_cross_jobs<4, 0, G, E, 1>);
+}
+
+static const void *cross_group_kernel(const fsx_cross_group *g)
+{
+    if (g->mixed) {
+        if (g->G >= 4) return cross_jobs_kernel<cplx32, 4>(g->tst, g->jn, g->pw);
+        if (g->G >= 2) return cross_jobs_kernel<cplx32, 2>(g->tst, g->jn, g->pw);
+        return cross_jobs_kernel<cplx32, 1>(g->tst, g->jn, g->pw);
+    }
+    if (g->G >= 4) return cross_jobs_kernel<cplx, 4>(g->tst, g->jn, g->pw);
+    if (g->G >= 2) return cross_jobs_kernel<cplx, 2>(g->tst, g->jn, g->pw);
+    return cross_jobs_kernel<cplx, 1>(g->tst, g->jn, g->pw);
+}
+
+/* the jobs of one complex's chunks appended to their groups (groups grown
+ * as needed); nx is set at launch */
+template <typename E, int G>
+static int cross_collect_g(fsx_cross_group **gr, int *ngr, int C, const int *csz, const int *cst,
+                           const long *coff, int Tn, int nrows, int Q, int L, int nb, int npk0,
+                           int ng, int zn, void *const *g)
+{
+    typedef decltype(E::re) F;
+    const int tst = (long)Tn * (long)(sizeof(E) / 2) <= 40 * 1024;
+    const long ntiles = (long)Q * nb * ((ng + G - 1) / G);
+    for (int c = 0; c < C; c++) {
+        if (csz[c] == 0) continue;
+        const int npk = c == 0 ? npk0 : 0;
+        const int ns = csz[c] - npk;
+        const int pw = sizeof(E) == 8 && ns <= 4 * (FSX_WAVE / 2) &&
+                       env_int("FSX_CROSS_PW", 2, 1, 2) == 2 ? 2 : 1;
+        const int jn = (ns + FSX_WAVE / pw - 1) / (FSX_WAVE / pw);
+        const size_t lds = sizeof(E) * (size_t)(cross_sb<E, G>() + 3 * G) * csz[c] +
+                           (tst ? cross_tst_bytes<F>(Tn) : 0) +
+                           (sizeof(F) * 6 + sizeof(int2)) * FSX_CROSS_PB;
+        const int mixed = sizeof(E) == 8, jq = jn <= 1 ? 1 : jn <= 2 ? 2 : jn <= 3 ? 3 : 4;
+        fsx_cross_group *t = NULL;
+        for (int i = 0; i < *ngr; i++) {
+            fsx_cross_group *u = &(*gr)[i];
+            if (u->mixed == mixed && u->G == G && u->tst == tst && u->jn == jq && u->pw == pw &&
+                u->chunk == c)
+                t = u;
+        }
+        if (!t) {
+            fsx_cross_group *ng2 = (fsx_cross_group *)realloc(*gr, sizeof(fsx_cross_group) * (*ngr + 1));
+            if (!ng2) return FSX_ERR_NOMEM;
+            *gr = ng2;
+            t = &(*gr)[(*ngr)++];
+            memset(t, 0, sizeof(*t));
+            t->mixed = mixed; t->G = G; t->tst = tst; t->jn = jq; t->pw = pw; t->chunk = c;
+        }
+        if (t->njobs == t->cap) {
+            const int cap = t->cap ? 2 * t->cap : 8;
+            fsx_cross_job *nj = (fsx_cross_job *)realloc(t->jobs, sizeof(fsx_cross_job) * cap);
+            if (!nj) return FSX_ERR_NOMEM;
+            t->jobs = nj;
+            t->cap = cap;
+        }
+        if (lds > t->lds) t->lds = lds;
+        fsx_cross_job *jb = &t->jobs[t->njobs++];
+        jb->acc = (double *)g[G_ACC];
+        jb->Rc = (const E *)g[G_RT] + coff[c];
+        jb->Bc = (const E *)g[G_BT] + coff[c];
+        jb->T = g[G_T];
+        jb->tw2 = g[G_TW2];
+        jb->ent = (const int *)g[G_ENT];
+        jb->counts = (const int *)g[G_COUNTS];
+        jb->offs = (const int *)g[G_OFFS];
+        jb->recs = (const int4 *)g[G_RECS];
+        jb->ntiles = ntiles;
+        jb->cstart = cst[c]; jb->nc = csz[c]; jb->nrows = nrows; jb->Q = Q; jb->L = L;
+        jb->nbeta = nb; jb->nx = 1; jb->ng = ng; jb->zn = zn; jb->first = c == 0; jb->npk = npk;
+    }
+    return FSX_OK;
+}
+
 /* G = gamma2 rows per tile: tile_rows when set (the batch tuner's choice),
  * else FSX_CROSS_ROWS, else 4 for FP32 tables (the translation block and the
  * pose loop shared by more rows) and 2 for FP64 tables (LDS-bound) */
@@ -2578,7 +2763,14 @@ static int plan_upload(fsx_plan *p, hipStream_t s, int up_dw, int up_ds)
 /* Every stage of the plan on stream s, no waiting; ev (may be NULL): seven
  * events recorded at the stage boundaries (after the tables, rotations,
  * bucketing, cross terms, fit preparation and fit). */
-static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_after_cross = 0)
+/* stages of plan_enqueue */
+#define FSX_ST_PREP 1      /* translation tables, rotated tables, pose buckets */
+#define FSX_ST_CROSS 2     /* cross terms */
+#define FSX_ST_FITPREP 4   /* fit queue and counters, profile rescale */
+#define FSX_ST_FIT 8       /* the fit */
+#define FSX_ST_ALL 15
+
+static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stages = FSX_ST_ALL)
 {
     const fsx_hip_args *A = &p->A;
     const int L = p->L, K = p->K, K2 = p->K2, N = p->N, Q = p->Q, n = p->n;
@@ -2587,7 +2779,9 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
     const int *csz = p->ent + 4 * K2, *cst = p->ent + 6 * K2;
     const long *coff = p->coff;
     void **g = p->d;
+    int *counts = (int *)g[P_COUNTS], *offs = (int *)g[P_OFFS], *fill = (int *)g[P_FILL];
     (void)na;
+    if (stages & FSX_ST_PREP) {
     /* translation tables */
     const long nbes = (long)zn * Q * N;
     hipLaunchKernelGGL(k_bessel, dim3(nblk(nbes, 256)), dim3(256), 0, s, (double *)g[P_BES],
@@ -2654,7 +2848,6 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[2], s));
     /* bucket poses by (z, beta2, gamma2) */
-    int *counts = (int *)g[P_COUNTS], *offs = (int *)g[P_OFFS], *fill = (int *)g[P_FILL];
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, counts, nkeys, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nkeys, 256)), dim3(256), 0, s, fill, nkeys, 0);
     hipLaunchKernelGGL(k_count, dim3(nblk(n, 256)), dim3(256), 0, s, p->dposes, n, zn, nb, p->na,
@@ -2664,7 +2857,9 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
                        (const int2 *)g[P_INFO], n, (const int *)offs, fill, (int4 *)g[P_RECS]);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[3], s));
+    }
     /* per-pose cross terms, one launch per m-chunk, accumulated in order */
+    if (stages & FSX_ST_CROSS) {
     const long grid = (long)g_ncu * env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
     if (p->mixed)
         launch_cross<cplx32>(s, C, csz, cst, coff, p->Tn, grid, nrows, Q, L, nb, p->npk, ng, zn, g,
@@ -2674,7 +2869,8 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
                            p->tile_rows);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[4], s));
-    if (stop_after_cross) return FSX_OK;
+    }
+    if (stages & FSX_ST_FITPREP) {
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_NEXT], 1L, (int)p->fit_next0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int *)g[P_TALLY], 2L * FSX_TALLY_N, 0);
     /* rescaled profiles in place of the pose parts (bounded by n slots;
@@ -2687,6 +2883,8 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
                        (const int *)(offs + nkeys), Q, (const double *)g[P_FITSC], p->dcross);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
+    }
+    if (stages & FSX_ST_FIT) {
     hipLaunchKernelGGL(p->kfit, dim3((unsigned)(p->lanes / 256)), dim3(256), p->lds_fit, s,
                        (const double *)g[P_ACC], (const int4 *)g[P_RECS],
                        (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
@@ -2695,6 +2893,7 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stop_aft
                        p->fit_wide);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[6], s));
+    }
     return FSX_OK;
 }
 
@@ -2840,11 +3039,15 @@ static int g_nbat = 0;
  * inputs uploaded before the stages run, outputs and the counters read
  * back afterwards are per complex */
 static fsx_bset g_bstr[FSX_BSTREAMS];
-static inline int per_stream_buffer(int i)
+/* persist: the persistent launches read the translation and rotated tables,
+ * the buckets, the accumulators and the fit queue of every complex after all
+ * complexes' table stages, so those are per complex */
+static inline int per_stream_buffer(int i, int persist)
 {
     switch (i) {
-    case P_BES: case P_T: case P_RT: case P_BT: case P_RTMP: case P_KEYS: case P_INFO:
-    case P_COUNTS: case P_FILL: case P_RECS: case P_ACC: case P_FITSC: case P_NEXT: case P_SRCT:
+    case P_T: case P_RT: case P_BT: case P_COUNTS: case P_RECS: case P_ACC: case P_NEXT:
+        return !persist;
+    case P_BES: case P_RTMP: case P_KEYS: case P_INFO: case P_FILL: case P_FITSC: case P_SRCT:
         return 1;
     default:
         return 0;
@@ -2852,6 +3055,7 @@ static inline int per_stream_buffer(int i)
 }
 struct fsx_bctx {
     fsx_bset *cplx, *strm;
+    int persist;
 };
 static hipStream_t g_bs[FSX_BSTREAMS];
 static hipEvent_t g_bfork, g_bjoin[FSX_BSTREAMS];
@@ -2897,7 +3101,7 @@ static void graph_drop(void)
 static int get_bat(int i, size_t bytes, void *ctx, void **out)
 {
     const fsx_bctx *c = (const fsx_bctx *)ctx;
-    fsx_bset *b = per_stream_buffer(i) ? c->strm : c->cplx;
+    fsx_bset *b = per_stream_buffer(i, c->persist) ? c->strm : c->cplx;
     if (b->cap[i] < bytes) {
         if (b->d[i]) HIPCHK(hipFree(b->d[i]));
         b->d[i] = NULL;
@@ -2925,9 +3129,18 @@ static int get_bat(int i, size_t bytes, void *ctx, void **out)
 #define FSX_TUNE_BUCKETS 6
 static int g_tune[FSX_TUNE_BUCKETS][2][2];   /* rows, slots; 0: not tuned */
 static int g_tune_loaded = 0;
+static fsx_cross_job *g_cjob_dev;
+static size_t g_cjob_cap;
+static fsx_fit_job *g_fjob_dev;
+static size_t g_fjob_cap;
 static void batch_release(void)
 {
     graph_drop();
+    if (g_cjob_dev) (void)hipFree(g_cjob_dev);
+    if (g_fjob_dev) (void)hipFree(g_fjob_dev);
+    g_cjob_dev = NULL;
+    g_fjob_dev = NULL;
+    g_cjob_cap = g_fjob_cap = 0;
     memset(g_tune, 0, sizeof(g_tune));
     g_tune_loaded = 0;
     for (int k = 0; k < g_nbat; k++)
@@ -3012,7 +3225,7 @@ static int tune_one(const fsx_hip_args *A)
         float ms = 0.f;
         for (int rep = 0; rc == FSX_OK && rep < 2; rep++) {
             if (hipEventRecord(ev[0], g_stream) != hipSuccess) rc = FSX_ERR_HIP;
-            if (rc == FSX_OK) rc = plan_enqueue(&p, g_stream, ev, 1);
+            if (rc == FSX_OK) rc = plan_enqueue(&p, g_stream, ev, FSX_ST_PREP | FSX_ST_CROSS);
             if (rc == FSX_OK && hipStreamSynchronize(g_stream) != hipSuccess) rc = FSX_ERR_HIP;
             if (rc == FSX_OK && hipEventElapsedTime(&ms, ev[3], ev[4]) != hipSuccess) rc = FSX_ERR_HIP;
         }
@@ -3031,6 +3244,38 @@ static int tune_one(const fsx_hip_args *A)
     return rc;
 }
 
+
+/* the cross-term jobs of plan p appended to the groups (the tile rows of
+ * launch_cross) */
+static int plan_cross_collect(const fsx_plan *p, fsx_cross_group **gr, int *ngr)
+{
+    const int G = p->tile_rows > 0 ? p->tile_rows
+                                   : env_int("FSX_CROSS_ROWS", p->mixed ? 4 : 2, 1, 4);
+    const int *csz = p->ent + 4 * p->K2, *cst = p->ent + 6 * p->K2;
+    void *const *g = p->d;
+#define FSX_COLLECT(E, GG)                                                                   \
+    cross_collect_g<E, GG>(gr, ngr, p->C, csz, cst, p->coff, p->Tn, p->nrows, p->Q, p->L,    \
+                           p->nb, p->npk, p->ng, p->zn, g)
+    if (p->mixed) return G >= 4 ? FSX_COLLECT(cplx32, 4) : G >= 2 ? FSX_COLLECT(cplx32, 2)
+                                                                : FSX_COLLECT(cplx32, 1);
+    return G >= 4 ? FSX_COLLECT(cplx, 4) : G >= 2 ? FSX_COLLECT(cplx, 2) : FSX_COLLECT(cplx, 1);
+#undef FSX_COLLECT
+}
+
+static int grow_dev(void **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return FSX_OK;
+    if (*buf) HIPCHK(hipFree(*buf));
+    *buf = NULL;
+    *cap = 0;
+    if (hipMalloc(buf, bytes) != hipSuccess) {
+        *buf = NULL;
+        return FSX_ERR_NOMEM;
+    }
+    *cap = bytes;
+    return FSX_OK;
+}
+
 static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
 {
     RC(init_once());
@@ -3038,6 +3283,16 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
     tune_load();
     const int use_graph = env_int("FSX_BATCH_GRAPH", 1, 0, 1);
     const int tune = env_int("FSX_TUNE", 1, 0, 1);
+    /* FSX_BATCH_PERSIST=1: the persistent form, one cross-term launch per
+     * (m-chunk index, kernel shape) and one fit launch over all complexes,
+     * after every complex's table stages.  Default: each complex's whole
+     * pipeline on one of four streams, which on the 64-complex bench batch
+     * is faster (674 against 873 ms per mixed step): the streams overlap one
+     * complex's tables, another's cross terms and a third's fit, where the
+     * persistent form runs those stages one after another (96 ms of tables,
+     * 601 ms of cross terms, 189 ms of fits, the last with a block-level
+     * tail per complex) */
+    const int persist = env_int("FSX_BATCH_PERSIST", 0, 0, 1);
     /* the tuner first (it runs in the process-wide buffers) */
     for (int i = 0; tune && i < nprob; i++) {
         const int m = (args[i].flags & FSX_MIXED_FP32) != 0, b = tune_bucket(args[i].L);
@@ -3054,7 +3309,13 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
     int *tail = (int *)calloc(2 * (size_t)nprob, sizeof(int));
     unsigned long long *tally =
         (unsigned long long *)calloc(FSX_TALLY_N * (size_t)nprob, sizeof(unsigned long long));
-    fsx_gkey *key = (fsx_gkey *)calloc(nprob > 0 ? nprob : 1, sizeof(fsx_gkey));
+    /* one key per complex and one for the batch's own launches */
+    fsx_gkey *key = (fsx_gkey *)calloc((size_t)nprob + 1, sizeof(fsx_gkey));
+    fsx_cross_group *cg = NULL;
+    int ncg = 0, nfj = 0;
+    fsx_fit_job *fj = NULL;
+    size_t fj_lds = 0;
+    long fj_lanes = 0;
     int rc = (pl && tail && tally && key) ? FSX_OK : FSX_ERR_NOMEM;
     int ninit = 0;
     for (; rc == FSX_OK && ninit < nprob; ninit++) {
@@ -3067,18 +3328,18 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
      * needs, before any plan takes their addresses */
     for (int j = 0; rc == FSX_OK && j < FSX_BSTREAMS; j++)
         for (int i = 0; rc == FSX_OK && i < P_NBUF; i++) {
-            if (!per_stream_buffer(i)) continue;
+            if (!per_stream_buffer(i, persist)) continue;
             size_t need = 0;
             for (int k = j; k < ninit; k += FSX_BSTREAMS)
                 if (pl[k].sz[i] > need) need = pl[k].sz[i];
             if (need == 0) continue;
-            fsx_bctx bc = {NULL, &g_bstr[j]};
+            fsx_bctx bc = {NULL, &g_bstr[j], persist};
             void *dummy;
             rc = get_bat(i, need, &bc, &dummy);
         }
     for (int k = 0; rc == FSX_OK && k < ninit; k++) {
         const fsx_hip_args *A = &args[k];
-        fsx_bctx bc = {&g_bat[k], &g_bstr[k % FSX_BSTREAMS]};
+        fsx_bctx bc = {&g_bat[k], &g_bstr[k % FSX_BSTREAMS], persist};
         rc = plan_bind(&pl[k], get_bat, &bc);
         if (rc != FSX_OK) break;
         fsx_bset *bs = &g_bat[k];
@@ -3089,6 +3350,94 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
             bs->dw_key = A->dw_key;
             bs->ds_key = A->ds_key;
         }
+    }
+    /* the persistent launches' jobs */
+    int fitjobs = 0;
+    if (rc == FSX_OK && persist && nprob > 0) {
+        for (int k = 0; rc == FSX_OK && k < ninit; k++)
+            if (pl[k].n > 0) rc = plan_cross_collect(&pl[k], &cg, &ncg);
+        /* launches in m-chunk order, so that each complex's chunks add up in
+         * order (one chunk of a complex per launch) */
+        for (int i = 1; i < ncg; i++)
+            for (int j = i; j > 0 && cg[j].chunk < cg[j - 1].chunk; j--) {
+                const fsx_cross_group t = cg[j];
+                cg[j] = cg[j - 1];
+                cg[j - 1] = t;
+            }
+        size_t ntot = 0;
+        for (int i = 0; i < ncg; i++) ntot += cg[i].njobs;
+        const long bpc_env = env_int("FSX_CROSS_BLOCKS_PER_CU", 0, 0, 16);
+        const int xcd = env_int("FSX_CROSS_XCD", 1, 0, 1);
+        for (int i = 0; rc == FSX_OK && i < ncg; i++) {
+            int per = (int)bpc_env;
+            if (per == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                 &per, cross_group_kernel(&cg[i]), FSX_CROSS_BLOCK, cg[i].lds) !=
+                                 hipSuccess ||
+                             per < 1))
+                per = 2;
+            const long gsz = (long)g_ncu * per;
+            for (int j = 0; j < cg[i].njobs; j++) {
+                fsx_cross_job *jb = &cg[i].jobs[j];
+                jb->nx = xcd && g_nxcd > 1 && gsz % g_nxcd == 0 && jb->ntiles >= 4L * gsz ? g_nxcd : 1;
+            }
+            cg[i].grid = gsz;
+        }
+        if (rc == FSX_OK) rc = grow_dev((void **)&g_cjob_dev, &g_cjob_cap,
+                                        sizeof(fsx_cross_job) * (ntot > 0 ? ntot : 1));
+        for (int i = 0, o = 0; rc == FSX_OK && i < ncg; o += cg[i].njobs, i++)
+            if (hipMemcpy(g_cjob_dev + o, cg[i].jobs, sizeof(fsx_cross_job) * cg[i].njobs,
+                          hipMemcpyHostToDevice) != hipSuccess)
+                rc = FSX_ERR_HIP;
+        /* one fit launch when every complex with poses takes the lane-group
+         * fit */
+        fitjobs = 1;
+        for (int k = 0; k < ninit; k++)
+            if (pl[k].n > 0 && pl[k].kfit != k_fitg) fitjobs = 0;
+        if (fitjobs && rc == FSX_OK) {
+            fj = (fsx_fit_job *)calloc((size_t)ninit, sizeof(fsx_fit_job));
+            if (!fj) rc = FSX_ERR_NOMEM;
+            for (int k = 0; rc == FSX_OK && k < ninit; k++) {
+                if (pl[k].n == 0) continue;
+                if (pl[k].lds_fit > fj_lds) fj_lds = pl[k].lds_fit;
+            }
+            int bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
+            if (rc == FSX_OK && bpc == 0 &&
+                (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fitg_jobs, 256, fj_lds) !=
+                     hipSuccess ||
+                 bpc < 1))
+                bpc = 1;
+            fj_lanes = (long)g_ncu * 256 * bpc;
+            for (int k = 0; rc == FSX_OK && k < ninit; k++) {
+                fsx_plan *p = &pl[k];
+                if (p->n == 0) continue;
+                const fsx_hip_args *A = &p->A;
+                void **g = p->d;
+                fsx_fit_job *f = &fj[nfj++];
+                f->acc = (const double *)g[P_ACC];
+                f->qvals = (const double *)g[P_Q];
+                f->a = (const double *)g[P_AM];
+                f->recs = (const int4 *)g[P_RECS];
+                f->nvalid_p = (const int *)g[P_OFFS] + p->nkeys;
+                f->mult = A->mult;
+                f->c1_lo = A->c1_lo; f->c1_hi = A->c1_hi; f->c2_lo = A->c2_lo; f->c2_hi = A->c2_hi;
+                f->score = p->dscore; f->c1o = p->dc1; f->c2o = p->dc2;
+                f->next = (int *)g[P_NEXT];
+                f->tally = (unsigned long long *)g[P_TALLY];
+                f->Q = p->Q;
+                f->fpl = p->fit_wide;
+                /* each job's queue starts past the groups the launch holds */
+                p->fit_next0 = fj_lanes / p->fit_wide;
+            }
+            if (rc == FSX_OK) rc = grow_dev((void **)&g_fjob_dev, &g_fjob_cap,
+                                            sizeof(fsx_fit_job) * (nfj > 0 ? nfj : 1));
+            if (rc == FSX_OK && nfj > 0 &&
+                hipMemcpy(g_fjob_dev, fj, sizeof(fsx_fit_job) * nfj, hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                rc = FSX_ERR_HIP;
+        }
+    }
+    for (int k = 0; rc == FSX_OK && k < ninit; k++) {
+        const fsx_hip_args *A = &args[k];
         fsx_gkey *ky = &key[k];
         const fsx_plan *p = &pl[k];
         ky->L = p->L; ky->nb = p->nb; ky->na = p->na; ky->ng = p->ng; ky->Q = p->Q; ky->zn = p->zn;
@@ -3100,24 +3449,66 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
         ky->dposes = p->dposes; ky->dscore = p->dscore; ky->dc1 = p->dc1; ky->dc2 = p->dc2;
         ky->dcross = p->dcross;
     }
+    if (rc == FSX_OK) {
+        /* the batch's own launches: the form, the job tables and their grids */
+        fsx_gkey *ky = &key[nprob];
+        ky->n = persist;
+        ky->flags = fitjobs;
+        ky->lanes = fj_lanes;
+        ky->d[0] = g_cjob_dev;
+        ky->d[1] = g_fjob_dev;
+        ky->C = ncg;
+        ky->Q = nfj;
+    }
     /* the stages of every complex: one graph launch, or the launches */
     if (rc == FSX_OK && nprob > 0) {
-        const size_t klen = sizeof(fsx_gkey) * nprob;
+        const size_t klen = sizeof(fsx_gkey) * ((size_t)nprob + 1);
         const int replay = use_graph && g_gexec && g_gkey_len == klen && memcmp(g_gkey, key, klen) == 0;
         if (!replay) {
             graph_drop();
             hipError_t e = use_graph ? hipStreamBeginCapture(g_stream, hipStreamCaptureModeRelaxed)
                                      : hipSuccess;
-            if (e == hipSuccess) e = hipEventRecord(g_bfork, g_stream);
-            for (int j = 0; e == hipSuccess && j < FSX_BSTREAMS; j++)
-                e = hipStreamWaitEvent(g_bs[j], g_bfork, 0);
+            /* fork to the streams, stages, join back on g_stream */
+            auto fork = [&]() {
+                hipError_t r = hipEventRecord(g_bfork, g_stream);
+                for (int j = 0; r == hipSuccess && j < FSX_BSTREAMS; j++)
+                    r = hipStreamWaitEvent(g_bs[j], g_bfork, 0);
+                return r;
+            };
+            auto join = [&]() {
+                hipError_t r = hipSuccess;
+                for (int j = 0; r == hipSuccess && j < FSX_BSTREAMS; j++) {
+                    r = hipEventRecord(g_bjoin[j], g_bs[j]);
+                    if (r == hipSuccess) r = hipStreamWaitEvent(g_stream, g_bjoin[j], 0);
+                }
+                return r;
+            };
+            if (e == hipSuccess) e = fork();
             if (e != hipSuccess) rc = FSX_ERR_HIP;
             for (int i = 0; rc == FSX_OK && i < nprob; i++)
-                rc = plan_enqueue(&pl[i], g_bs[i % FSX_BSTREAMS], NULL);
-            for (int j = 0; rc == FSX_OK && j < FSX_BSTREAMS; j++) {
-                if (hipEventRecord(g_bjoin[j], g_bs[j]) != hipSuccess ||
-                    hipStreamWaitEvent(g_stream, g_bjoin[j], 0) != hipSuccess)
-                    rc = FSX_ERR_HIP;
+                rc = plan_enqueue(&pl[i], g_bs[i % FSX_BSTREAMS], NULL,
+                                  persist ? FSX_ST_PREP : FSX_ST_ALL);
+            if (rc == FSX_OK && join() != hipSuccess) rc = FSX_ERR_HIP;
+            if (persist) {
+                for (int i = 0, o = 0; rc == FSX_OK && i < ncg; o += cg[i].njobs, i++) {
+                    const fsx_cross_job *jobs = g_cjob_dev + o;
+                    int nj = cg[i].njobs;
+                    void *kargs[2] = {(void *)&jobs, (void *)&nj};
+                    if (hipLaunchKernel(cross_group_kernel(&cg[i]), dim3((unsigned)cg[i].grid),
+                                        dim3(FSX_CROSS_BLOCK), kargs, cg[i].lds, g_stream) !=
+                        hipSuccess)
+                        rc = FSX_ERR_HIP;
+                }
+                if (rc == FSX_OK && fork() != hipSuccess) rc = FSX_ERR_HIP;
+                for (int i = 0; rc == FSX_OK && i < nprob; i++)
+                    rc = plan_enqueue(&pl[i], g_bs[i % FSX_BSTREAMS], NULL,
+                                      FSX_ST_FITPREP | (fitjobs ? 0 : FSX_ST_FIT));
+                if (rc == FSX_OK && join() != hipSuccess) rc = FSX_ERR_HIP;
+                if (rc == FSX_OK && fitjobs && nfj > 0) {
+                    hipLaunchKernelGGL(k_fitg_jobs, dim3((unsigned)(fj_lanes / 256)), dim3(256),
+                                       fj_lds, g_stream, (const fsx_fit_job *)g_fjob_dev, nfj);
+                    if (hipGetLastError() != hipSuccess) rc = FSX_ERR_HIP;
+                }
             }
             if (use_graph) {
                 hipGraph_t gr = NULL;
@@ -3149,6 +3540,9 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
         plan_stats(&pl[i], &st[i], NULL, tail + 2 * i, tally + FSX_TALLY_N * i);
     if (rc != FSX_OK) graph_drop();
     for (int i = 0; i < ninit; i++) plan_free_host(&pl[i]);
+    for (int i = 0; i < ncg; i++) free(cg[i].jobs);
+    free(cg);
+    free(fj);
     free(pl);
     free(tail);
     free(tally);

@@ -5,9 +5,11 @@ launches of the same batch (FSX_BATCH_GRAPH=0) and against single calls,
 with the program's own eager work (single fsx_score calls, which use other
 device buffers) and synchronisations between the replays; a replay with
 new pose contents in the same buffers must give the new poses' results.
-The tile-shape tuner's table is written to FSX_TUNE_FILE.  Bit-identical
-throughout: the graph replays the same kernels on the same data, and the
-tuned tile rows change no arithmetic."""
+The tile-shape tuner's table is written to FSX_TUNE_FILE.  The persistent
+launches (one cross-term launch per m-chunk and kernel shape, one fit launch,
+over all complexes) are checked against each complex's own pipeline.
+Bit-identical throughout: the graph replays the same kernels on the same
+data, and the tuned tile rows change no arithmetic."""
 import os
 
 import numpy as np
@@ -81,3 +83,37 @@ def test_graph_replays_match_eager(F, tmp_path, monkeypatch):
     assert sorted((int(l[0]), int(l[1])) for l in lines) == [(0, 0), (0, 1), (1, 0), (1, 1),
                                                               (2, 0), (2, 1)]
     assert all(int(l[2]) in (1, 2, 4) for l in lines)
+
+
+def test_persistent_launches_match_per_complex(F, monkeypatch):
+    """The persistent form (FSX_BATCH_PERSIST=1: one cross-term launch per
+    m-chunk index and kernel shape over all complexes, one fit launch over
+    all complexes' pose queues) against the per-complex form (the default,
+    FSX_BATCH_PERSIST=0: each complex's whole pipeline on one of four
+    streams), eager and replayed, in both forms.  The complexes cover one
+    and two m-chunks (L = 20 in the mixed form), two tile shapes and
+    q grids of 11 to 70 points (lane groups of 2 to 8 lanes per fit): the
+    same kernels on the same data, so bit-identical."""
+    pbs = [build_problem(300, 200, 10, 0.4, 21, np.arange(22.0, 28.0), 150, seed=61),
+           build_problem(400, 300, 20, 0.5, 17, np.arange(30.0, 34.0), 80, seed=62),
+           build_problem(400, 250, 15, 0.5, 70, np.arange(25.0, 31.0), 100, seed=63, nbeta=19,
+                         na=64, ng=64),
+           build_problem(200, 150, 6, 0.3, 11, np.arange(20.0, 24.0), 40, seed=64)]
+    gps = [_gp(F, pb) for pb in pbs]
+    poses = [pb["index"] for pb in pbs]
+    for mixed in (False, True):
+        res = {}
+        for persist in ("0", "1"):
+            for graph in ("0", "1"):
+                monkeypatch.setenv("FSX_BATCH_PERSIST", persist)
+                monkeypatch.setenv("FSX_BATCH_GRAPH", graph)
+                print(f"mixed={mixed} persist={persist} graph={graph}", flush=True)
+                res[(persist, graph)] = F.score_batch(gps, poses, want_cross=True, mixed=mixed)
+        base = res[("0", "0")]
+        for k, rb in res.items():
+            for r, q in zip(rb, base):
+                assert _same(r, q), (mixed, k)
+                assert r["stats"]["n_scored"] == q["stats"]["n_scored"]
+                assert r["stats"]["fit_evals"] == q["stats"]["fit_evals"]
+        for gp, p, r in zip(gps, poses, base):
+            assert _same(r, F.score(gp, p, want_cross=True, mixed=mixed))
