@@ -155,6 +155,40 @@ def run_batch(args):
         "mixed_over_fp64": mixed["value"] / fp64["value"], "cpu_baseline": None}))
 
 
+def run_atoms(args):
+    """The atom expansion (fsx_atoms2spf, pdb2spf.c:24-152) of the large
+    workload's receptor (10,000 atoms with water shell, L = 25, 81 q on
+    [0, 0.6]) in both forms: FP64 MFMA sums and the split-BF16 MFMA sums of
+    the mixed form.  Host arrays in and out (the call uploads the atoms and
+    downloads the amplitudes); reported per call, with the largest difference
+    between the forms relative to each (X, q) block's largest amplitude."""
+    import libfmftsaxs_amd as F
+    from libfmftsaxs_amd import synthetic as S
+    wk = WORKLOADS["large"]
+    rec, _ = S.make_dimer_inputs(wk["n_rec"], 300, 1001, 2001, water=True)
+    q = F.mkarray(0.0, wk["qmax"], wk["qnum"])
+    L = wk["L"]
+    res = {}
+    for mixed in (False, True):
+        for _ in range(args.warmup):
+            F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L, mixed=mixed)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = F.atoms2spf(rec.coords, rec.vac, rec.dum, rec.sa, rec.vdw, q, L, mixed=mixed)
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res["mixed" if mixed else "fp64"] = (ms, out)
+        log(f"{'mixed' if mixed else 'fp64'} atom expansion: {ms:.2f} ms per call")
+    (r64, i64, _), (r16, i16, _) = res["fp64"][1], res["mixed"][1]
+    scale = np.maximum(np.abs(r64).max(axis=2, keepdims=True), np.abs(i64).max(axis=2, keepdims=True))
+    err = float((np.maximum(np.abs(r16 - r64), np.abs(i16 - i64)) / scale).max())
+    print(json.dumps({"metric": "atom expansion (fsx_atoms2spf) per call, host arrays in and out",
+                      "config": {"workload": f"{rec.natoms} atoms with water shell, L = {L}, "
+                                             f"{wk['qnum']} q on [0, {wk['qmax']}]"},
+                      "steps": args.steps, "warmup": args.warmup,
+                      "fp64_ms": res["fp64"][0], "mixed_ms": res["mixed"][0],
+                      "mixed_max_rel_error": err}))
+
+
 def _oracle_rate(w, sample, nthr):
     import oracle as O
     O.set_pi(O.PI_TRUNC)
@@ -253,10 +287,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--poses", type=int, default=0, help="pose count (default: the workload's)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch"], default="standard",
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "atoms"], default="standard",
                     help="standard (the headline workload), large (L = 25 single complex), "
-                         "small (the CPU-runnable case) or batch (many complexes in one "
-                         "call)")
+                         "small (the CPU-runnable case), batch (many complexes in one "
+                         "call) or atoms (the atom expansion in both forms)")
     ap.add_argument("--complexes", type=int, default=64, help="complexes of the batch workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true",
@@ -287,6 +321,9 @@ def main():
         raise SystemExit("no HIP device")
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "atoms":
+        run_atoms(args)
         return
     wk = dict(WORKLOADS[args.workload])
     if args.poses:

@@ -158,6 +158,17 @@ int fsx_atoms2spf(int natoms, const double *coords, const double *vac, const dou
                   const double *sa, double h2o_ff, const double *vdw, const double *qvals,
                   int qnum, int L, double *re, double *im, double *rm);
 
+/* fsx_atoms2spf in either form: flags 0 (the sums over atoms as FP64 MFMA
+ * products, within 1e-12 of the reference's FP64 loop) or FSX_MIXED_FP32
+ * (the mixed-precision form: each j_l(q r) and f conj(Y_lm) split into three
+ * BF16 parts, the six leading part products on BF16 MFMA with FP32
+ * accumulation over each 1024-atom chunk, the chunks summed in FP64; about
+ * 1e-6 of each (X, q, l) block's largest amplitude).  No counterpart in the
+ * reference, which has one (FP64) form. */
+int fsx_atoms2spf_form(int natoms, const double *coords, const double *vac, const double *dum,
+                       const double *sa, double h2o_ff, const double *vdw, const double *qvals,
+                       int qnum, int L, unsigned flags, double *re, double *im, double *rm);
+
 /* Allocation-free form of fsx_atoms2spf.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph
  * capture, serving loops, harnesses that forbid allocation).  Every pointer

@@ -102,6 +102,10 @@ def lib():
         L.fsx_atoms2spf.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double, _dp, _dp,
                                     C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.fsx_atoms2spf.restype = C.c_int
+        L.fsx_atoms2spf_form.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double, _dp,
+                                         _dp, C.c_int, C.c_int, C.c_uint, _dp, _dp,
+                                         C.POINTER(C.c_double)]
+        L.fsx_atoms2spf_form.restype = C.c_int
         L.fsx_score_naive.argtypes = ([C.c_int, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_double,
                                        _dp, _dp, C.c_int, C.c_int, _dp] + [C.c_double] * 5 +
                                       [_dp, C.c_int, _dp, C.c_void_p])
@@ -290,18 +294,21 @@ def sbessel_batch(x, pmax):
     return out
 
 
-def atoms2spf(coords, vac, dum, sa, vdw, qvals, L, h2o_ff=3.50968):
+def atoms2spf(coords, vac, dum, sa, vdw, qvals, L, h2o_ff=3.50968, mixed=False):
     """GPU atom expansion (pdb2spf.c:24-152).  Returns (re, im, rm) with
-    re/im shaped [3, Q, (L+1)^2]."""
+    re/im shaped [3, Q, (L+1)^2].  mixed: the sums over atoms in split BF16
+    on MFMA with FP32 accumulation (fsx_atoms2spf_form, FSX_MIXED_FP32)."""
     coords = f64(coords).reshape(-1, 3)
     qvals = f64(qvals)
     re = np.zeros((3, qvals.size, (L + 1) ** 2))
     im = np.zeros_like(re)
     rm = C.c_double(0.0)
     sa_arr = None if sa is None else f64(sa)
-    _chk(lib().fsx_atoms2spf(coords.shape[0], coords.ravel(), f64(vac), f64(dum),
-                             None if sa_arr is None else sa_arr.ctypes.data, h2o_ff, f64(vdw),
-                             qvals, qvals.size, L, re.reshape(-1), im.reshape(-1), C.byref(rm)))
+    _chk(lib().fsx_atoms2spf_form(coords.shape[0], coords.ravel(), f64(vac), f64(dum),
+                                  None if sa_arr is None else sa_arr.ctypes.data, h2o_ff,
+                                  f64(vdw), qvals, qvals.size, L,
+                                  FSX_MIXED_FP32 if mixed else 0, re.reshape(-1), im.reshape(-1),
+                                  C.byref(rm)))
     return re, im, rm.value
 
 

@@ -47,9 +47,11 @@ int fsx_hip_fit_batch(const double *ct, int n, const double *qvals, int qnum, co
                       double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
                       double *out);
 int fsx_hip_sbessel_batch(const double *x, int n, int pmax, double *out);
+/* bf16: the sums over atoms in split BF16 on MFMA (the mixed form) instead
+ * of FP64 MFMA */
 int fsx_hip_atoms2spf(int natoms, const double *xyz, const double *vac, const double *dum,
                       const double *sa, double h2o_ff, const double *qvals, int qnum, int L,
-                      double *re, double *im);
+                      double *re, double *im, int bf16);
 int fsx_hip_device_count(void);
 void fsx_hip_release(void);
 

@@ -2016,6 +2016,148 @@ k_atom_mfma(const double *J, const double *f, const double *Y, double *P, int na
     }
 }
 
+/* ---- the atom expansion in split BF16 on MFMA (the mixed form) ----------
+ * The same sums over atoms as k_atom_mfma, each FP64 operand x written as
+ * three BF16 parts x = h + m + l + O(2^-24 x) (h = bf16(x), m = bf16(x - h),
+ * l = bf16(x - h - m)), and every product as the six part products of at
+ * least that order (hh, hm, mh, hl, lh, mm) on v_mfma_f32_16x16x32_bf16 with
+ * FP32 accumulation; the chunk partials go to FP64 and k_atom_finish sums
+ * them in order.  The parts are formed once per value by k_atom_bes_split
+ * (j_l(q r), [part][l][q][atom]) and k_atom_bsplit (f^X conj(Y_lm),
+ * [part][column][atom]), atoms innermost so that a lane's 8 atoms of a step
+ * are one 16-byte load; atoms are padded to whole chunks with zeros. */
+typedef __bf16 fsx_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float fsx_v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split3(double x, __bf16 &h, __bf16 &m, __bf16 &l)
+{
+    h = (__bf16)(float)x;
+    const double r1 = x - (double)(float)h;
+    m = (__bf16)(float)r1;
+    const double r2 = r1 - (double)(float)m;
+    l = (__bf16)(float)r2;
+}
+
+/* Js[part][l][q][a] = parts of j_l(q r_a), 0 for q >= Q or a >= na */
+__global__ void k_atom_bes_split(__bf16 *Js, const double *r, const double *qvals, int na, int nap,
+                                 int Q, int Qp, int K)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)K * Qp * nap;
+    if (i >= tot) return;
+    const int a = (int)(i % nap);
+    const long lq = i / nap;
+    const int q = (int)(lq % Qp), l = (int)(lq / Qp);
+    const double v = q < Q && a < na ? dev_sbessel(l, qvals[q] * r[a]) : 0.0;
+    __bf16 h, m, lo;
+    split3(v, h, m, lo);
+    Js[i] = h;
+    Js[tot + i] = m;
+    Js[2 * tot + i] = lo;
+}
+
+/* Bs[part][3 l (l+1) + c][a] = parts of f^X_a conj(Y_lm)(a) for column c =
+ * X * 2 (l+1) + 2 m + (0 real, 1 imaginary) of order l */
+__global__ void k_atom_bsplit(__bf16 *Bs, const double *f, const double *Y, int na, int nap, int L)
+{
+    const int K = L + 1, Kh = K * (K + 1) / 2, ncolt = 6 * Kh;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)ncolt * nap;
+    if (i >= tot) return;
+    const int a = (int)(i % nap), cg = (int)(i / nap);
+    int l = 0;
+    while (3 * (l + 1) * (l + 2) <= cg) l++;
+    const int c = cg - 3 * l * (l + 1);
+    const int X = c / (2 * (l + 1)), rem = c - X * 2 * (l + 1);
+    const double v = a < na ? f[(size_t)X * na + a] * Y[(size_t)a * Kh * 2 + l * (l + 1) / 2 * 2 + rem]
+                            : 0.0;
+    __bf16 h, m, lo;
+    split3(v, h, m, lo);
+    Bs[i] = h;
+    Bs[tot + i] = m;
+    Bs[2 * tot + i] = lo;
+}
+
+/* One block of four waves per (chunk, l, 16-q tile), as k_atom_mfma.  BF16
+ * MFMA 16x16x32 operand layout on gfx950: lane i feeds A[i % 16][8 (i / 16)
+ * + e] and B[8 (i / 16) + e][i % 16], e = 0..7; result register j of lane i
+ * is D[4 (i / 16) + j][i % 16].  Writes the chunk's partial sums
+ * P[chunk][X][Qp][Kh][2] in FP64. */
+__global__ void __launch_bounds__(256)
+k_atom_bf16(const __bf16 *Js, const __bf16 *Bs, double *P, int nap, int Qp, int L, int chunk0)
+{
+    __shared__ fsx_v4f red[3][64];
+    const int K = L + 1, Kh = K * (K + 1) / 2, nqt = Qp / 16;
+    int bid = blockIdx.x;
+    const int qt = bid % nqt;
+    bid /= nqt;
+    const int l = bid % K;
+    const int ch = bid / K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ncol = 3 * 2 * (l + 1), nct = (ncol + 15) / 16;
+    const long jtot = (long)K * Qp * nap, btot = (long)6 * Kh * nap;
+    fsx_v4f acc[FSX_AT_MAXT];
+#pragma unroll
+    for (int t = 0; t < FSX_AT_MAXT; t++) acc[t] = (fsx_v4f){0.f, 0.f, 0.f, 0.f};
+    const int abeg = ch * FSX_AT_CHUNK + w * (FSX_AT_CHUNK / 4);
+    const __bf16 *Ja = Js + ((long)l * Qp + qt * 16 + (lane & 15)) * nap + 8 * (lane >> 4);
+    const __bf16 *Bl = Bs + (long)3 * l * (l + 1) * nap + 8 * (lane >> 4);
+    for (int a0 = abeg; a0 < abeg + FSX_AT_CHUNK / 4; a0 += 32) {
+        const fsx_bf16x8 ah = *reinterpret_cast<const fsx_bf16x8 *>(Ja + a0);
+        const fsx_bf16x8 am = *reinterpret_cast<const fsx_bf16x8 *>(Ja + jtot + a0);
+        const fsx_bf16x8 al = *reinterpret_cast<const fsx_bf16x8 *>(Ja + 2 * jtot + a0);
+#pragma unroll
+        for (int t = 0; t < FSX_AT_MAXT; t++) {
+            if (t < nct) {
+                const int c = t * 16 + (lane & 15);
+                fsx_bf16x8 bh = {}, bm = {}, bl = {};
+                if (c < ncol) {
+                    const __bf16 *Bc = Bl + (long)c * nap + a0;
+                    bh = *reinterpret_cast<const fsx_bf16x8 *>(Bc);
+                    bm = *reinterpret_cast<const fsx_bf16x8 *>(Bc + btot);
+                    bl = *reinterpret_cast<const fsx_bf16x8 *>(Bc + 2 * btot);
+                }
+                fsx_v4f v = acc[t];
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, v, 0, 0, 0);
+                acc[t] = v;
+            }
+        }
+    }
+    /* wave 0 + wave 1 + wave 2 + wave 3, in that order, per tile */
+    const int q0 = qt * 16;
+    double *Pc = P + (size_t)(chunk0 + ch) * 3 * Qp * Kh * 2;
+#pragma unroll
+    for (int t = 0; t < FSX_AT_MAXT; t++) {
+        if (t < nct) {
+            if (w > 0) red[w - 1][lane] = acc[t];
+            __syncthreads();
+            if (w == 0) {
+                const int c = t * 16 + (lane & 15);
+                const int X = c / (2 * (l + 1)), rem = c - X * 2 * (l + 1);
+                double v[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    v[j] = (double)acc[t][j] + (double)red[0][lane][j] + (double)red[1][lane][j] +
+                           (double)red[2][lane][j];
+                if (c < ncol) {
+                    const int coff = l * (l + 1) / 2 * 2 + rem;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int q = q0 + 4 * (lane >> 4) + j;
+                        Pc[((size_t)X * Qp + q) * Kh * 2 + coff] = v[j];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 /* A^X_lm = 4 pi i^l S_lm and A^X_{l,-m} = (-1)^m 4 pi i^l conj(S_lm), S the
  * sum of the nch chunk partials in chunk order; one thread per (X, q, l, m >= 0) */
 __global__ void k_atom_finish(const double *P, int nch, double *re, double *im, int Q, int Qp,
@@ -3667,13 +3809,52 @@ static int atoms_run(hipStream_t s, char *ws, size_t ws_bytes, int natoms, const
     return hipGetLastError() == hipSuccess ? FSX_OK : FSX_ERR_HIP;
 }
 
+/* The mixed form of atoms_run in one pass over all atoms: the BF16 parts of
+ * the Bessel values and of f conj(Y_lm), then the split products on BF16
+ * MFMA.  Workspace: the chunk partials, then per atom (padded to whole
+ * chunks) f, r, Y in FP64 and the parts. */
+static inline size_t at_bf16_atom_bytes(int Q, int L)
+{
+    const size_t K = L + 1, Kh = K * (K + 1) / 2;
+    return sizeof(double) * (4 + 2 * Kh) + 3 * sizeof(__bf16) * (K * at_qp(Q) + 6 * Kh);
+}
+static int atoms_run_bf16(hipStream_t s, char *ws, size_t ws_bytes, int natoms,
+                          const double *xyz, const double *vac, const double *dum,
+                          const double *sa, double h2o_ff, const double *qv, int Q, int L,
+                          double *re, double *im)
+{
+    const int K = L + 1, Kh = K * (K + 1) / 2, Qp = at_qp(Q);
+    const int nch = (natoms + FSX_AT_CHUNK - 1) / FSX_AT_CHUNK, nap = nch * FSX_AT_CHUNK;
+    const size_t pb = at_part_bytes(natoms, Q, L);
+    if (ws_bytes < pb + at_bf16_atom_bytes(Q, L) * (size_t)nap) return FSX_ERR_ARG;
+    double *P = reinterpret_cast<double *>(ws);
+    double *f = reinterpret_cast<double *>(ws + pb), *r = f + 3 * (size_t)natoms;
+    double *Y = r + natoms;
+    __bf16 *Js = reinterpret_cast<__bf16 *>(Y + (size_t)natoms * Kh * 2);
+    __bf16 *Bs = Js + 3 * (size_t)K * Qp * nap;
+    hipLaunchKernelGGL(k_atom_geom, dim3(nblk(natoms, 256)), dim3(256), 0, s, xyz, vac, dum, sa,
+                       h2o_ff, natoms, L, FSX_PI_REFERENCE, r, f, Y);
+    hipLaunchKernelGGL(k_atom_bes_split, dim3(nblk((long)K * Qp * nap, 256)), dim3(256), 0, s, Js,
+                       (const double *)r, qv, natoms, nap, Q, Qp, K);
+    hipLaunchKernelGGL(k_atom_bsplit, dim3(nblk(6L * Kh * nap, 256)), dim3(256), 0, s, Bs,
+                       (const double *)f, (const double *)Y, natoms, nap, L);
+    hipLaunchKernelGGL(k_atom_bf16, dim3((unsigned)(nch * K * (Qp / 16))), dim3(256), 0, s,
+                       (const __bf16 *)Js, (const __bf16 *)Bs, P, nap, Qp, L, 0);
+    hipLaunchKernelGGL(k_atom_finish, dim3(nblk(3L * Q * Kh, 256)), dim3(256), 0, s,
+                       (const double *)P, nch, re, im, Q, Qp, L, 4.0 * FSX_PI_REFERENCE);
+    return hipGetLastError() == hipSuccess ? FSX_OK : FSX_ERR_HIP;
+}
+
 extern "C" int fsx_hip_atoms2spf(int natoms, const double *xyz, const double *vac,
                                  const double *dum, const double *sa, double h2o_ff,
-                                 const double *qvals, int Q, int L, double *re, double *im)
+                                 const double *qvals, int Q, int L, double *re, double *im,
+                                 int bf16)
 {
     pthread_mutex_lock(&g_mtx);
     const int K = L + 1, K2 = K * K;
-    const size_t wsb = at_part_bytes(natoms, Q, L) + at_atom_bytes(Q, L) * (size_t)natoms;
+    const size_t nap = ((size_t)natoms + FSX_AT_CHUNK - 1) / FSX_AT_CHUNK * FSX_AT_CHUNK;
+    const size_t wsb = at_part_bytes(natoms, Q, L) +
+                       (bf16 ? at_bf16_atom_bytes(Q, L) * nap : at_atom_bytes(Q, L) * (size_t)natoms);
     int rc = init_once();
     if (rc == FSX_OK) rc = ensure(B_AXYZ, sizeof(double) * 3 * (size_t)natoms);
     if (rc == FSX_OK) rc = ensure(B_AFF, sizeof(double) * 3 * (size_t)natoms);
@@ -3691,11 +3872,10 @@ extern "C" int fsx_hip_atoms2spf(int natoms, const double *xyz, const double *va
         if (e == hipSuccess && sa) e = hipMemcpyAsync(dff + 2 * (size_t)natoms, sa, nb, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(g_buf[B_AQ], qvals, sizeof(double) * Q, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
-            rc = atoms_run(s, (char *)g_buf[B_AWS], g_cap[B_AWS], natoms,
-                           (const double *)g_buf[B_AXYZ], dff, dff + natoms,
-                           sa ? dff + 2 * (size_t)natoms : NULL, h2o_ff,
-                           (const double *)g_buf[B_AQ], Q, L, (double *)g_buf[B_ARE],
-                           (double *)g_buf[B_AIM]);
+            rc = (bf16 ? atoms_run_bf16 : atoms_run)(
+                s, (char *)g_buf[B_AWS], g_cap[B_AWS], natoms, (const double *)g_buf[B_AXYZ], dff,
+                dff + natoms, sa ? dff + 2 * (size_t)natoms : NULL, h2o_ff,
+                (const double *)g_buf[B_AQ], Q, L, (double *)g_buf[B_ARE], (double *)g_buf[B_AIM]);
             if (rc == FSX_ERR_HIP) e = hipErrorUnknown;
         }
         if (e == hipSuccess && rc == FSX_OK)
