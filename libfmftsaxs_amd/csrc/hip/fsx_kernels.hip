@@ -1142,8 +1142,11 @@ __global__ void k_fit_scale(const double *acc, const double *cint, const int *nv
  * pose part, and the unscaled profiles to cross_out when asked; one thread
  * per (pose slot, q): its six terms, contiguous and 16-byte aligned, are read
  * and written as three pairs. */
+/* accf (the mixed form's lane-group fit): the rescaled profiles in FP32 to
+ * accf [pose][q][6] instead of in place */
 __global__ void k_fit_prep(double *acc, const double *cint, const int4 *recs,
-                           const int *nvalid_p, int Q, const double *scale, double *cross_out)
+                           const int *nvalid_p, int Q, const double *scale, double *cross_out,
+                           float *accf)
 {
 #pragma clang fp contract(off)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1169,6 +1172,12 @@ __global__ void k_fit_prep(double *acc, const double *cint, const int4 *recs,
         for (int k = 0; k < 6; k++) cross_out[o + (long)k * Q] = x[k] * 1.0;
     }
     const double sc = scale[s];
+    if (accf) {
+        float2 *f2 = reinterpret_cast<float2 *>(accf + i * 6);
+#pragma unroll
+        for (int h = 0; h < 3; h++) f2[h] = make_float2((float)(x[2 * h] * sc), (float)(x[2 * h + 1] * sc));
+        return;
+    }
 #pragma unroll
     for (int h = 0; h < 3; h++) a2[h] = make_double2(x[2 * h] * sc, x[2 * h + 1] * sc);
 }
@@ -1654,7 +1663,7 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
 #define FSX_FG_WAVES 3   /* wavefronts per SIMD the register budget allows */
 #endif
 
-__device__ __forceinline__ void fitg_eval(const double *pr, const double *sNt, int ns, int qpl,
+__device__ __forceinline__ void fitg_eval(const float *pr, const double *sNt, int ns, int qpl,
                                           int fpl, int j, int Q, double mult, double a0sum,
                                           const double *x, double *fv, double *g)
 {
@@ -1679,8 +1688,8 @@ __device__ __forceinline__ void fitg_eval(const double *pr, const double *sNt, i
         for (int k = 0; k < 6; k++) s[k] += tt[k];
     };
     for (int ii = 0; ii < nq; ii++) {
-        const double2 *p2 = reinterpret_cast<const double2 *>(pr + 6 * (q0 + ii));
-        const double2 a = p2[0], b = p2[1], c = p2[2];
+        const float2 *p2 = reinterpret_cast<const float2 *>(pr + 6 * (q0 + ii));
+        const float2 a = p2[0], b = p2[1], c = p2[2];
         const double cur[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
         /* I and its derivatives in c1, c2 (min_saxs.c:52-68) */
         const double qc = sNt[6 * ns + ii * fpl + j];
@@ -1711,7 +1720,7 @@ __device__ __forceinline__ void fitg_eval(const double *pr, const double *sNt, i
 }
 
 __device__ __forceinline__ void
-fitg_body(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+fitg_body(const float *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
           const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
           double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
 {
@@ -1740,7 +1749,7 @@ fitg_body(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const
     unsigned long long nev = 0, ncap = 0, nmax = 0;
     fsx_lb2 *S = sSt + threadIdx.x / fpl;
     while (s < nvalid) {
-        const double *pr = acc + (long)s * 6 * Q;
+        const float *pr = acc + (long)s * 6 * Q;
         fsx_lb2_start(S, &fp, 1.0, 0.0);   /* define.h:33-34 */
         int more;
         do {
@@ -1782,7 +1791,7 @@ fitg_body(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const
 }
 
 __global__ void __launch_bounds__(256, FSX_FG_WAVES)
-k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
+k_fitg(const float *acc, const int4 *recs, const int *nvalid_p, int Q, const double *qvals,
        const double *a, double mult, double c1_lo, double c1_hi, double c2_lo, double c2_hi,
        double *score, double *c1o, double *c2o, int *next, unsigned long long *tally, int fpl)
 {
@@ -1795,7 +1804,8 @@ k_fitg(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const do
  * empty, then those of job 1, and so on (each job with its own queue, q
  * grid, moments, bounds and lanes per fit). */
 struct fsx_fit_job {
-    const double *acc, *qvals, *a;
+    const float *acc;
+    const double *qvals, *a;
     const int4 *recs;
     const int *nvalid_p;
     double mult, c1_lo, c1_hi, c2_lo, c2_hi;
@@ -2210,7 +2220,7 @@ enum {
     B_BES, B_T, B_RT, B_BT, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_POSES, B_KEYS, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_INFO, B_ACC, B_SCORE, B_C1, B_C2,
     B_CROSSOUT, B_ENT, B_COFF, B_NEXT, B_TALLY, B_SRCT, B_TW2, B_FITCT, B_FITOUT, B_BX, B_BOUT, B_AXYZ,
-    B_AFF, B_AWS, B_AQ, B_ARE, B_AIM, B_RTMP, B_FITSC, B_NBUF
+    B_AFF, B_AWS, B_AQ, B_ARE, B_AIM, B_RTMP, B_FITSC, B_ACCF, B_NBUF
 };
 static void *g_buf[B_NBUF];
 static size_t g_cap[B_NBUF];
@@ -2596,13 +2606,13 @@ static int spf_conj_symmetric(const double *re, const double *im, int Q, int L)
 enum {
     P_BES, P_T, P_RT, P_BT, P_RTMP, P_A, P_B, P_DW, P_DS, P_TWA, P_TWG, P_Q, P_Z, P_AM, P_CINT,
     P_KEYS, P_INFO, P_COUNTS, P_OFFS, P_FILL, P_RECS, P_ACC, P_FITSC, P_ENT, P_TW2, P_COFF,
-    P_NEXT, P_TALLY, P_SRCT, P_POSES, P_SCORE, P_C1, P_C2, P_CROSSOUT, P_NBUF
+    P_NEXT, P_TALLY, P_SRCT, P_POSES, P_SCORE, P_C1, P_C2, P_CROSSOUT, P_ACCF, P_NBUF
 };
 /* the process-wide slot of each plan buffer (single calls) */
 static const int k_plan_slot[P_NBUF] = {
     B_BES, B_T, B_RT, B_BT, B_RTMP, B_A, B_B, B_DW, B_DS, B_TWA, B_TWG, B_Q, B_Z, B_AM, B_CINT,
     B_KEYS, B_INFO, B_COUNTS, B_OFFS, B_FILL, B_RECS, B_ACC, B_FITSC, B_ENT, B_TW2, B_COFF,
-    B_NEXT, B_TALLY, B_SRCT, B_POSES, B_SCORE, B_C1, B_C2, B_CROSSOUT};
+    B_NEXT, B_TALLY, B_SRCT, B_POSES, B_SCORE, B_C1, B_C2, B_CROSSOUT, B_ACCF};
 
 typedef void (*fit_kernel_t)(const double *, const int4 *, const int *, int, const double *,
                              const double *, double, double, double, double, double, double *,
@@ -2623,7 +2633,8 @@ struct fsx_plan {
     int aq_lds;
     size_t lds_fit;
     long lanes;
-    fit_kernel_t kfit;
+    fit_kernel_t kfit;   /* the compact-form fit (grp = 0) */
+    int grp;             /* the lane-group fit k_fitg (mixed form) */
     int fit_wide;        /* lanes with a fit at or below which a fit wavefront
                             evaluates them together (L-BFGS-B fits); lanes per
                             fit (Newton fits) */
@@ -2802,12 +2813,13 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     p->kfit = p->aq_lds ? (p->mixed ? k_fit<1, 1> : k_fit<1, 0>)
                         : (p->mixed ? k_fit<0, 1> : k_fit<0, 0>);
     p->fit_wide = env_int("FSX_FIT_WIDE", 24, 0, 64);
+    p->grp = grp;
     if (grp) {
-        p->kfit = k_fitg;
         p->fit_wide = fpl;
         p->lds_fit = fitg_lds(Q, fpl);
+        p->sz[P_ACCF] = sizeof(float) * 6 * Q * (size_t)n;
         int bpc = env_int("FSX_FIT_BLOCKS_PER_CU", 0, 0, 16);
-        if (bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, p->kfit, 256,
+        if (bpc == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fitg, 256,
                                                                       p->lds_fit) != hipSuccess ||
                          bpc < 1))
             bpc = 1;
@@ -3022,17 +3034,26 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stages =
                        (double *)g[P_FITSC]);
     hipLaunchKernelGGL(k_fit_prep, dim3(nblk((long)n * Q, 256)), dim3(256), 0, s,
                        (double *)g[P_ACC], (const double *)g[P_CINT], (const int4 *)g[P_RECS],
-                       (const int *)(offs + nkeys), Q, (const double *)g[P_FITSC], p->dcross);
+                       (const int *)(offs + nkeys), Q, (const double *)g[P_FITSC], p->dcross,
+                       p->grp ? (float *)g[P_ACCF] : (float *)NULL);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[5], s));   /* the fit kernel alone from here */
     }
     if (stages & FSX_ST_FIT) {
-    hipLaunchKernelGGL(p->kfit, dim3((unsigned)(p->lanes / 256)), dim3(256), p->lds_fit, s,
-                       (const double *)g[P_ACC], (const int4 *)g[P_RECS],
-                       (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
-                       (const double *)g[P_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo, A->c2_hi,
-                       p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT], (unsigned long long *)g[P_TALLY],
-                       p->fit_wide);
+    if (p->grp)
+        hipLaunchKernelGGL(k_fitg, dim3((unsigned)(p->lanes / 256)), dim3(256), p->lds_fit, s,
+                           (const float *)g[P_ACCF], (const int4 *)g[P_RECS],
+                           (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
+                           (const double *)g[P_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo,
+                           A->c2_hi, p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT],
+                           (unsigned long long *)g[P_TALLY], p->fit_wide);
+    else
+        hipLaunchKernelGGL(p->kfit, dim3((unsigned)(p->lanes / 256)), dim3(256), p->lds_fit, s,
+                           (const double *)g[P_ACC], (const int4 *)g[P_RECS],
+                           (const int *)(offs + nkeys), Q, (const double *)g[P_Q],
+                           (const double *)g[P_AM], A->mult, A->c1_lo, A->c1_hi, A->c2_lo,
+                           A->c2_hi, p->dscore, p->dc1, p->dc2, (int *)g[P_NEXT],
+                           (unsigned long long *)g[P_TALLY], p->fit_wide);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[6], s));
     }
@@ -3188,6 +3209,7 @@ static inline int per_stream_buffer(int i, int persist)
 {
     switch (i) {
     case P_T: case P_RT: case P_BT: case P_COUNTS: case P_RECS: case P_ACC: case P_NEXT:
+    case P_ACCF:
         return !persist;
     case P_BES: case P_RTMP: case P_KEYS: case P_INFO: case P_FILL: case P_FITSC: case P_SRCT:
         return 1;
@@ -3534,7 +3556,7 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
          * fit */
         fitjobs = 1;
         for (int k = 0; k < ninit; k++)
-            if (pl[k].n > 0 && pl[k].kfit != k_fitg) fitjobs = 0;
+            if (pl[k].n > 0 && !pl[k].grp) fitjobs = 0;
         if (fitjobs && rc == FSX_OK) {
             fj = (fsx_fit_job *)calloc((size_t)ninit, sizeof(fsx_fit_job));
             if (!fj) rc = FSX_ERR_NOMEM;
@@ -3555,7 +3577,7 @@ static int run_batch_locked(int nprob, const fsx_hip_args *args, fsx_stats *st)
                 const fsx_hip_args *A = &p->A;
                 void **g = p->d;
                 fsx_fit_job *f = &fj[nfj++];
-                f->acc = (const double *)g[P_ACC];
+                f->acc = (const float *)g[P_ACCF];
                 f->qvals = (const double *)g[P_Q];
                 f->a = (const double *)g[P_AM];
                 f->recs = (const int4 *)g[P_RECS];

@@ -2,7 +2,8 @@
 2 x 2 L-BFGS matrix held explicitly, one pose per lane group, the profile in
 registers) against the same fit compiled for the host (fsx_fit_host_form
 with FSX_MIXED_FP32), fed the device's own cross-term profiles, rescaled as
-min_saxs.c:170-179 does.  The two differ only in the order of the sums over
+min_saxs.c:170-179 does and rounded to FP32 as the mixed form stores them for
+its fit.  The two differ only in the order of the sums over
 q (a lane's q block, then the butterfly across the group), so chi must agree
 to 1e-9 relative and c1, c2 to 1e-5.  The q grids cover one register block
 per lane with 8 lanes per fit (Q = 25), 64 lanes (Q = 150) and the form that
@@ -25,9 +26,10 @@ def F():
 
 
 def _rescaled(ct, peak):
-    """min_saxs.c:170-179: every term times peak / I(q_0) at c1 = 1, c2 = 1"""
+    """min_saxs.c:170-179: every term times peak / I(q_0) at c1 = 1, c2 = 1,
+    rounded to FP32 as the mixed form stores the profiles its fit reads"""
     s = ct[0, 0] + ct[3, 0] + ct[5, 0] + ct[2, 0] - ct[1, 0] - ct[4, 0]
-    return ct * (peak / s)
+    return (ct * (peak / s)).astype(np.float32).astype(np.float64)
 
 
 @pytest.mark.parametrize("L,Q,npose,seed", [(8, 25, 400, 71), (6, 150, 200, 72),
