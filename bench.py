@@ -403,7 +403,9 @@ def main():
                 roof_cross.update({"bound": "hbm", "achieved": ab, "peak": 8000.0,
                                    "peak_basis": "HBM3E", "unit": "GB/s", "frac": ab / 8000.0,
                                    "valu_tflops": achieved, "valu_frac": achieved / peak})
-        fb = n * (6 * pb.qnum * 8 + 24)
+        # each pose's six rescaled profile terms read once (FP32 in the mixed
+        # form's lane-group fit, FP64 otherwise) and score, c1, c2 written
+        fb = n * (6 * pb.qnum * (4 if mixed else 8) + 24)
         achieved = fb / (avg["ms_fit"] * 1e-3) / 1e9
         roof_fit = {"kernel": "k_fit", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
                     "unit": "GB/s", "frac": achieved / 8000.0,
