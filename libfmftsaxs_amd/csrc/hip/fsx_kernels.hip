@@ -750,9 +750,9 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         const E *Rq = Rc + (long)q * nrows * 3 * nc;   /* receptor rows of this q */
         bool staged = false;
         /* Each z's bucket header, translation block (TST) and first pose
-         * records are read one z ahead: the next z's header is requested at
-         * the top of an iteration, its translation block before the U step
-         * and its records after it, so that they arrive while this z's U
+         * records are read one z ahead: the next z's header is requested
+         * before the U step, its translation block and records after it,
+         * so that they arrive while this z's U
          * step and pose loop run instead of stalling the block between z
          * values.  Header: the tile's buckets are adjacent keys, their poses
          * adjacent in recs; lane r < nr of every wavefront reads the count
@@ -801,14 +801,13 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         rload(start, cnt);
         for (int z = 0; z < zn; z++) {
             const bool zmore = z + 1 < zn;
-            const int hv = zmore ? hload(z + 1) : 0;
             int cum2[G + 1], start2 = 0;
 #pragma unroll
             for (int r = 0; r <= G; r++) cum2[r] = 0;
             if (cnt == 0) {
                 /* an empty z: straight on to the next one */
                 if (zmore) {
-                    hread(hv, cum2, start2);
+                    hread(hload(z + 1), cum2, start2);
                     const int cnt2 = cum2[G];
                     if (TST && cnt2 > 0) tdma(z + 1);
                     rload(start2, cnt2);
@@ -858,6 +857,10 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
              * landed in sT (the barrier before the U step makes it visible
              * to the other wavefronts) */
             if (TST) __builtin_amdgcn_s_waitcnt(FSX_WAITCNT_VM0);
+            /* the next z's header, requested after that wait (which would
+             * otherwise wait for it too) and read after the U step */
+            const int hv = zmore ? hload(z + 1) : 0;
+            __builtin_amdgcn_sched_barrier(0);
 
             if (tid < cnt && tid < FSX_CROSS_PB) sP[tid] = rpf;
             /* staging: four loads per thread in flight (an index past the end
