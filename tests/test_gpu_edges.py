@@ -1,0 +1,101 @@
+"""Edge shapes of the HIP scoring path against the oracle: the smallest
+expansion orders (L = 1..4, down to three translation entries per row and
+two pairing slots), a single pose, a single z plane, and alpha / gamma
+grids of one point.  Bars as in the parity and mixed-form tests: the FP64
+form within 1e-10 relative in I(q), scores within 1e-7, c1 / c2 within 1e-6
+and the identical top ranking; the mixed form within 1e-5 in I(q), chi
+within 1e-4 and the same ranking up to exact ties.  Q >= 3 throughout: with
+two q points the fitted objective of some poses rounds below zero in the
+oracle itself (chi = NaN), which says nothing about the path."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.problem import build_problem
+from tests.test_gpu_mixed import same_ranking
+from tests.test_gpu_parity import _I, _gpu_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def F():
+    import libfmftsaxs_amd as F
+    if F.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return F
+
+
+def _check_fp64(r, ro):
+    """The parity bars of the FP64 form, with the ranking compared up to
+    exact ties (at small orders distinct grid poses can be images of one
+    another and score the same)."""
+    ok = ~np.isnan(ro["score"])
+    assert np.array_equal(ok, ~np.isnan(r["score"]))
+    cg, co = r["cross"][ok], ro["cross"][ok]
+    Ig, Io = _I(cg), _I(co)
+    assert (np.abs(Ig - Io) / np.abs(Io)).max() <= 1e-10
+    assert (np.abs(cg - co) / np.abs(co).max(axis=2, keepdims=True)).max() <= 1e-10
+    s_g, s_o = r["score"][ok], ro["score"][ok]
+    assert np.all(np.abs(s_g - s_o) <= 1e-7 * np.abs(s_o))
+    assert np.all(np.abs(r["c1"][ok] - ro["c1"][ok]) <= 1e-6)
+    assert np.all(np.abs(r["c2"][ok] - ro["c2"][ok]) <= 1e-6)
+    assert same_ranking(s_g, s_o, min(100, int(ok.sum())))
+
+
+def _check_mixed(r, ro):
+    ok = ~np.isnan(ro["score"])
+    assert np.array_equal(ok, ~np.isnan(r["score"]))
+    Ig, Io = _I(r["cross"][ok]), _I(ro["cross"][ok])
+    assert (np.abs(Ig - Io) / np.abs(Io)).max() <= 1e-5
+    assert (np.abs(r["score"][ok] - ro["score"][ok]) / ro["score"][ok]).max() <= 1e-4
+    assert same_ranking(r["score"][ok], ro["score"][ok], min(100, int(ok.sum())))
+
+
+@pytest.mark.parametrize("L,Q,zn,npose", [(1, 4, 3, 24), (2, 5, 3, 40), (3, 9, 2, 60),
+                                          (4, 12, 4, 60)])
+def test_small_orders_vs_oracle(F, L, Q, zn, npose):
+    pb = build_problem(200, 150, L, 0.3, Q, np.arange(20.0, 20.0 + zn), npose, seed=70 + L)
+    ro = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                               pb["params"], pb["qvals"], pb["zvals"], L, want_cross=True)
+    assert not np.isnan(ro["score"]).any()
+    gp = _gpu_problem(F, pb)
+    print(f"L={L} Q={Q} zn={zn} n={npose}: FP64 form", flush=True)
+    _check_fp64(F.score(gp, pb["index"], want_cross=True), ro)
+    print(f"L={L} Q={Q} zn={zn} n={npose}: mixed form", flush=True)
+    _check_mixed(F.score(gp, pb["index"], want_cross=True, mixed=True), ro)
+
+
+def test_single_pose_and_single_plane(F):
+    L = 6
+    pb = build_problem(300, 200, L, 0.3, 21, np.array([24.0]), 30, seed=90)
+    ro = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                               pb["params"], pb["qvals"], pb["zvals"], L, want_cross=True)
+    gp = _gpu_problem(F, pb)
+    print("one z plane, 30 poses", flush=True)
+    _check_fp64(F.score(gp, pb["index"], want_cross=True), ro)
+    for k in (0, 17):
+        print(f"one pose (pose {k})", flush=True)
+        one = pb["index"][k:k + 1]
+        r1 = F.score(gp, one, want_cross=True)
+        r1o = {key: ro[key][k:k + 1] for key in ("score", "c1", "c2", "cross")}
+        _check_fp64(r1, r1o)
+        assert np.array_equal(r1["score"], F.score(gp, pb["index"], want_cross=True)["score"][k:k + 1])
+
+
+@pytest.mark.parametrize("na,ng,nb,npose", [(1, 1, 2, 8), (1, 5, 3, 20), (4, 1, 2, 20)])
+def test_one_point_angle_grids_vs_oracle(F, na, ng, nb, npose):
+    """alpha2 or gamma grids of a single point (every pose at angle 0), and
+    two or three beta values."""
+    L = 4
+    O.set_pi(O.PI_TRUNC)
+    pb = build_problem(200, 150, L, 0.3, 9, np.arange(20.0, 23.0), npose, seed=80, nbeta=nb,
+                       na=na, ng=ng)
+    ro = O.compute_saxs_scores(pb["index"], pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                               pb["params"], pb["qvals"], pb["zvals"], L, nbeta=nb, na=na, ng=ng,
+                               want_cross=True, dense_threshold=1)
+    gp = _gpu_problem(F, pb)
+    print(f"na={na} ng={ng} nbeta={nb}: FP64 form", flush=True)
+    _check_fp64(F.score(gp, pb["index"], want_cross=True), ro)
+    print(f"na={na} ng={ng} nbeta={nb}: mixed form", flush=True)
+    _check_mixed(F.score(gp, pb["index"], want_cross=True, mixed=True), ro)
