@@ -99,3 +99,26 @@ def test_one_point_angle_grids_vs_oracle(F, na, ng, nb, npose):
     _check_fp64(F.score(gp, pb["index"], want_cross=True), ro)
     print(f"na={na} ng={ng} nbeta={nb}: mixed form", flush=True)
     _check_mixed(F.score(gp, pb["index"], want_cross=True, mixed=True), ro)
+
+
+def test_batch_with_small_and_empty_complexes(F):
+    """fsx_score_batch over an L = 1 complex, an empty pose list, an L = 3
+    complex and a single pose: every non-empty complex gets exactly the
+    results of its own fsx_score call, in both forms."""
+    pbs = [build_problem(200, 150, 1, 0.3, 4, np.arange(20.0, 23.0), 24, seed=71),
+           build_problem(200, 150, 3, 0.3, 9, np.arange(20.0, 22.0), 60, seed=73),
+           build_problem(300, 200, 6, 0.3, 21, np.array([24.0]), 30, seed=90)]
+    gps = [_gpu_problem(F, pb) for pb in pbs]
+    probs = [gps[0], gps[1], gps[1], gps[2]]
+    poses = [pbs[0]["index"], pbs[1]["index"][:0], pbs[1]["index"], pbs[2]["index"][3:4]]
+    for mixed in (False, True):
+        print(f"batch of 4 (one empty), mixed={mixed}", flush=True)
+        rb = F.score_batch(probs, poses, want_cross=True, mixed=mixed)
+        assert rb[1]["score"].size == 0
+        for gp, p, r in zip(probs, poses, rb):
+            if p.size == 0:
+                continue
+            rs = F.score(gp, p, want_cross=True, mixed=mixed)
+            for k in ("score", "c1", "c2", "cross"):
+                assert np.array_equal(r[k], rs[k], equal_nan=True), k
+            assert not np.isnan(r["score"]).any()
