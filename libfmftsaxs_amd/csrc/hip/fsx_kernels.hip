@@ -598,8 +598,8 @@ __global__ void k_scan(const int *counts, int nk, int *offs)
 }
 
 /* sorted pose records: x = input position, y = receptor row, z = alpha index */
-__global__ void k_scatter(const int *keys, const int2 *info, int n, const int *offs, int *fill,
-                          int4 *recs)
+__global__ void k_scatter(const int *keys, const int2 *info, int n, int ng, const int *offs,
+                          int *fill, int4 *recs)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -607,7 +607,8 @@ __global__ void k_scatter(const int *keys, const int2 *info, int n, const int *o
     if (k < 0) return;
     int p = offs[k] + atomicAdd(&fill[k], 1);
     int2 inf = info[i];
-    recs[p] = make_int4(i, inf.x, inf.y, 0);
+    /* input position, receptor row, alpha2, gamma2 (key % ng) */
+    recs[p] = make_int4(i, inf.x, inf.y, k % ng);
 }
 
 /* ---- per-pose cross terms, one m-chunk per launch --------------------------
@@ -651,6 +652,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
 {
 #pragma clang fp contract(fast)
     typedef decltype(E::re) F;   /* arithmetic type of the contraction */
+    static_assert(G <= 8, "the tile row of a pose is kept in three bits of its record");
     extern __shared__ double smem_raw[];
     E *smem = reinterpret_cast<E *>(smem_raw);
     /* ligand rows: FP32 tables [nc][SB] (the 3G values of one entry together,
@@ -662,7 +664,8 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
     F *sT = reinterpret_cast<F *>(sU + 3 * G * nc);   /* T block, [l1][l] per |m| as in T (TST) */
     const int K = L + 1, K2 = K * K, N = 2 * L + 1, Tn = tpk_sq(K);
     /* the current batch of up to FSX_CROSS_PB of the tile's poses: their
-     * (receptor row, alpha2), read from LDS in the pose loop so that a pose
+     * (receptor row, 8 alpha2 + gamma2 row within the tile), read from LDS
+     * in the pose loop so that a pose
      * waits for one global round trip (its receptor row), not two, and their
      * six lane sums (type F), written to acc by the whole block after the
      * batch, so that no store is in flight in the pose loop */
@@ -718,6 +721,13 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
      * sl, sl + LP, ... (J per lane) */
     constexpr int LP = FSX_WAVE / PW;
     const int sl = lane % LP, half = lane / LP;
+    /* bytes per receptor row (three parts of nc entries), the lane's slot
+     * within a row, and the step to its last slot (back to slot 0 when that
+     * one is past the last slot) */
+    const unsigned rstride = 3u * (unsigned)nc * (unsigned)sizeof(E);
+    const unsigned soff = (unsigned)(sl * sizeof(E));
+    const unsigned dlast = sl + LP * (J - 1) < ns ? (unsigned)(LP * (J - 1) * sizeof(E))
+                                                  : (unsigned)0 - soff;
     /* byte offset of each slot's twiddle e^(-i m alpha2) within a row of tw2 */
     unsigned emo[J];
 #pragma unroll
@@ -748,6 +758,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         const int nr = ng - g20 < G ? ng - g20 : G;   /* rows of this tile */
         const E *Bq = Bc + (((long)q * nrows + b2 * ng + g20) * 3) * nc;
         const E *Rq = Rc + (long)q * nrows * 3 * nc;   /* receptor rows of this q */
+        const char *RqB = reinterpret_cast<const char *>(Rq);
         bool staged = false;
         /* Each z's bucket header, translation block (TST) and first pose
          * records are read one z ahead: the next z's header is requested
@@ -791,7 +802,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         auto rload = [&](int st, int n) {
             if (tid < n && tid < FSX_CROSS_PB) {
                 const int4 r4 = recs[st + tid];
-                rpf = make_int2(r4.y, r4.z);
+                rpf = make_int2(r4.y, (r4.z << 3) | (r4.w - g20));
             }
         };
         int cum[G + 1], start = 0;
@@ -831,26 +842,32 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
             int prowc = 0;
             auto load_pose = [&](int p, int pb, E (&rr)[3][J], F (&wr)[J], F (&wi)[J],
                                  int &prow) {
-                prow = 0;
-#pragma unroll
-                for (int r = 1; r < G; r++) prow += p >= cum[r];
                 const int2 rec = sP[p - pb];
+                prow = rec.y & 7;
                 /* no branches, so that the J slots' loads are all in flight
                  * together: a lane's slot past the last one (only the last
                  * of the J can be) reads slot 0, and the pairing gives it a
-                 * zero twiddle */
-                const E *R = Rq + (long)rec.x * 3 * nc + sl;
-                const unsigned twr = (unsigned)rec.y * (unsigned)(N * sizeof(E));
+                 * zero twiddle.  Receptor entries: the part's base (uniform,
+                 * in scalar registers) plus one 32-bit byte offset of the
+                 * row and the lane's slot (the host keeps a q's rows below
+                 * 4 GiB and their count below 2^24), the slot step LP j as
+                 * the load's immediate offset */
+                const unsigned roff = __umul24((unsigned)rec.x, rstride) + soff;
+                const unsigned twr = __umul24((unsigned)rec.y >> 3, (unsigned)(N * sizeof(E)));
 #pragma unroll
                 for (int j = 0; j < J; j++) {
-                    const bool ok = j < J - 1 || sl + LP * j < ns;
-                    const E *Rj = ok ? R + LP * j : R - sl;
                     const E w = *reinterpret_cast<const E *>(
                         reinterpret_cast<const char *>(tw2) + (twr + emo[j]));
                     wr[j] = w.re;
                     wi[j] = w.im;
 #pragma unroll
-                    for (int X = 0; X < 3; X++) rr[X][j] = Rj[X * nc];
+                    for (int X = 0; X < 3; X++) {
+                        const char *base = RqB + (size_t)X * nc * sizeof(E);
+                        rr[X][j] = j < J - 1
+                                       ? *reinterpret_cast<const E *>(base + (size_t)roff +
+                                                                      (size_t)(LP * j * sizeof(E)))
+                                       : *reinterpret_cast<const E *>(base + (size_t)(roff + dlast));
+                    }
                 }
             };
             /* the translation block copied in the background (TST) has
@@ -966,7 +983,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
             if (pb > 0) {
                 if (tid < FSX_CROSS_PB && pb + tid < cnt) {
                     const int4 r4 = recs[start + pb + tid];
-                    sP[tid] = make_int2(r4.y, r4.z);
+                    sP[tid] = make_int2(r4.y, (r4.z << 3) | (r4.w - g20));
                 }
                 lds_barrier();
             }
@@ -976,7 +993,8 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
             auto pair_pose = [&](const E (&rr)[3][J], const F (&wr)[J], const F (&wi)[J],
                                  int prow, int p) {
                 const bool act = p < pe;
-                const E *sUp = sU + prow * 3 * nc;
+                const E *sUp = reinterpret_cast<const E *>(reinterpret_cast<const char *>(sU) +
+                                                           __umul24((unsigned)prow, rstride));
                 F va[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int j = 0; j < J; j++) {
@@ -2736,6 +2754,13 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
         plan_free_host(p);
         return FSX_ERR_ARG;
     }
+    /* the cross-term kernel addresses a q's receptor rows and a row of the
+     * twiddle table by 32-bit byte offsets from 24-bit products */
+    if ((size_t)p->nrows * 3 * ncmax * p->es >= ((size_t)1 << 32) || p->nrows >= (1 << 24) ||
+        na >= (1 << 24) || (size_t)na * (2 * L + 1) * p->es >= ((size_t)1 << 32)) {
+        plan_free_host(p);
+        return FSX_ERR_ARG;
+    }
     p->C = C;
     p->ne = ne;
     /* U-phase thread -> entry of each chunk: entries by increasing |m|
@@ -3094,7 +3119,7 @@ static int plan_enqueue(fsx_plan *p, hipStream_t s, hipEvent_t *ev, int stages =
                        ng, (int *)g[P_KEYS], (int2 *)g[P_INFO], counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const int *)counts, (int)nkeys, offs);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, (const int *)g[P_KEYS],
-                       (const int2 *)g[P_INFO], n, (const int *)offs, fill, (int4 *)g[P_RECS]);
+                       (const int2 *)g[P_INFO], n, ng, (const int *)offs, fill, (int4 *)g[P_RECS]);
     HIPR(hipGetLastError());
     if (ev) HIPR(hipEventRecord(ev[3], s));
     }
