@@ -737,9 +737,11 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         emo[j] = (unsigned)(m + L) * (unsigned)sizeof(E);
     }
     /* after the lane reduction: one pose: lanes 0, 8, .., 40 hold the six
-     * sums; two poses: lanes 0, 4, .., 20 of each half */
-    const int slot = PW == 1 ? lane >> 3 : (lane >> 2) & 7;
-    const bool writer = (PW == 1 ? (lane & 7) == 0 : (lane & 3) == 0) && slot < 6;
+     * sums; two poses: lanes 0, 8, 4, 16, 24, 20 of each half (bits 4, 3, 2
+     * of the lane select sum 3 b4 + (b2 ? 2 : b3)) */
+    const int rb4 = (lane >> 4) & 1, rb3 = (lane >> 3) & 1, rb2 = (lane >> 2) & 1;
+    const int slot = PW == 1 ? lane >> 3 : 3 * rb4 + (rb2 ? 2 : rb3);
+    const bool writer = PW == 1 ? (lane & 7) == 0 && slot < 6 : (lane & 3) == 0 && !(rb2 && rb3);
     const int ngg = (ng + G - 1) / G;   /* gamma2 groups per beta2 */
     const int nbg = nbeta * ngg;
     /* Tiles in nx contiguous ranges of the (q, beta2, gamma2 group) order,
@@ -1040,19 +1042,20 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
                     v[0] += dpp_mov<FSX_DPP_XOR1>(v[0]);
                     v[0] += dpp_mov<FSX_DPP_XOR2>(v[0]);
                 } else {
-                    /* the same within each 32-lane half (one pose each): after
-                     * the halving steps over lane bits 4 (16-row swap), 3
-                     * (row_ror:8) and 2 (row_half_mirror: lane j <-> 7 - j)
-                     * those bits select the slot, then a sum over the quad of
-                     * bits 1,0; lanes 0, 4, .., 20 of each half hold slots 0..5 */
+                    /* the same within each 32-lane half (one pose each), on
+                     * the six sums: lane bit 4 (16-row swap) pairs sums i and
+                     * i + 3; bit 3 (row_ror:8) pairs the first two of those
+                     * and adds the third across itself; bit 2
+                     * (row_half_mirror: lane j <-> 7 - j) pairs the two
+                     * results; then a sum over the quad of bits 1,0 */
 #pragma unroll
-                    for (int i = 0; i < 4; i++) v[i] = tr_swap16(v[i], v[i + 4]);
-                    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-                        const F send = b3 ? v[i] : v[i + 2];
-                        const F keep = b3 ? v[i + 2] : v[i];
-                        v[i] = keep + dpp_mov<FSX_DPP_ROR8>(send);
+                    for (int i = 0; i < 3; i++) v[i] = tr_swap16(v[i], v[i + 3]);
+                    const bool b3 = rb3 != 0, b2 = rb2 != 0;
+                    {
+                        const F send = b3 ? v[0] : v[1];
+                        const F keep = b3 ? v[1] : v[0];
+                        v[0] = keep + dpp_mov<FSX_DPP_ROR8>(send);
+                        v[1] = v[2] + dpp_mov<FSX_DPP_ROR8>(v[2]);
                     }
                     const F send = b2 ? v[0] : v[1];
                     const F keep = b2 ? v[1] : v[0];
