@@ -122,3 +122,37 @@ def test_batch_with_small_and_empty_complexes(F):
             for k in ("score", "c1", "c2", "cross"):
                 assert np.array_equal(r[k], rs[k], equal_nan=True), k
             assert not np.isnan(r["score"]).any()
+
+
+def test_dense_buckets_vs_oracle(F):
+    """Hundreds of poses in the buckets of one cross-term tile (one z, one
+    beta2, gamma2 rows 0..3), so that a tile's poses run in several batches
+    of the pose loop, followed by a z with a few poses: both forms against
+    the oracle."""
+    L = 6
+    pb = build_problem(300, 200, L, 0.3, 21, np.arange(20.0, 23.0), 10, seed=95)
+    nb, na, ng, zn = pb["nbeta"], pb["na"], pb["ng"], pb["zvals"].size
+    rng = np.random.default_rng(96)
+
+    def idx(z, b1, b2, a2, g1, g2):
+        return ((((z * nb + b1) * nb + b2) * na + a2) * ng + g1) * ng + g2
+
+    poses = []
+    for g2, m in ((2, 300), (0, 150), (3, 40), (1, 5)):
+        pick = rng.choice(nb * na * ng, size=m, replace=False)
+        b1, r = np.divmod(pick, na * ng)
+        a2, g1 = np.divmod(r, ng)
+        poses.append(idx(1, b1, 4, a2, g1, g2))
+    pick = rng.choice(nb * na * ng, size=7, replace=False)
+    b1, r = np.divmod(pick, na * ng)
+    a2, g1 = np.divmod(r, ng)
+    poses.append(idx(2, b1, 4, a2, g1, 2))
+    index = rng.permutation(np.concatenate(poses).astype(np.int64))
+    assert index.max() < zn * nb * nb * na * ng * ng
+    ro = O.compute_saxs_scores(index, pb["A_re"], pb["A_im"], pb["B_re"], pb["B_im"],
+                               pb["params"], pb["qvals"], pb["zvals"], L, want_cross=True)
+    gp = _gpu_problem(F, pb)
+    print(f"{index.size} poses, 495 in one tile: FP64 form", flush=True)
+    _check_fp64(F.score(gp, index, want_cross=True), ro)
+    print(f"{index.size} poses, 495 in one tile: mixed form", flush=True)
+    _check_mixed(F.score(gp, index, want_cross=True, mixed=True), ro)
