@@ -1085,7 +1085,12 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
                 *d = first ? x : *d + x;
             }
             }
-            lds_barrier();
+            /* no barrier here: until the next z's first barrier a wavefront
+             * only writes the next records (sP), the staged ligand rows (sB,
+             * at a new tile) and the translation block (sT, read by the U
+             * step before its closing barrier), none of which the batch's
+             * writing of sO reads; the next pose loop writes sO only after
+             * that barrier */
 #pragma unroll
             for (int r = 0; r <= G; r++) cum[r] = cum2[r];
             start = start2;
