@@ -728,12 +728,14 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
     const unsigned soff = (unsigned)(sl * sizeof(E));
     const unsigned dlast = sl + LP * (J - 1) < ns ? (unsigned)(LP * (J - 1) * sizeof(E))
                                                   : (unsigned)0 - soff;
-    /* byte offset of each slot's twiddle e^(-i m alpha2) within a row of tw2 */
+    /* byte offset of each slot's twiddle e^(-i m alpha2) within a row of tw2
+     * (rows of 2L + 2 entries, m = -L..L and a zero at 2L + 1, which a slot
+     * past the last one reads) */
     unsigned emo[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
         int ec = sl + LP * j;
-        const int m = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : 0;
+        const int m = ec < ns ? ent_m[cstart + (ec < npk ? 2 * ec : ec + npk)] : L + 1;
         emo[j] = (unsigned)(m + L) * (unsigned)sizeof(E);
     }
     /* after the lane reduction: one pose: lanes 0, 8, .., 40 hold the six
@@ -802,8 +804,8 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
         };
         int2 rpf = make_int2(0, 0);   /* a first record (tid < FSX_CROSS_PB) */
         auto rload = [&](int st, int n) {
-            if (tid < n && tid < FSX_CROSS_PB) {
-                const int4 r4 = recs[st + tid];
+            if (n > 0 && tid < FSX_CROSS_PB) {
+                const int4 r4 = recs[st + min(tid, n - 1)];
                 rpf = make_int2(r4.y, (r4.z << 3) | (r4.w - g20));
             }
         };
@@ -855,7 +857,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
                  * 4 GiB and their count below 2^24), the slot step LP j as
                  * the load's immediate offset */
                 const unsigned roff = __umul24((unsigned)rec.x, rstride) + soff;
-                const unsigned twr = __umul24((unsigned)rec.y >> 3, (unsigned)(N * sizeof(E)));
+                const unsigned twr = __umul24((unsigned)rec.y >> 3, (unsigned)((N + 1) * sizeof(E)));
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     const E w = *reinterpret_cast<const E *>(
@@ -881,7 +883,10 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
             const int hv = zmore ? hload(z + 1) : 0;
             __builtin_amdgcn_sched_barrier(0);
 
-            if (tid < cnt && tid < FSX_CROSS_PB) sP[tid] = rpf;
+            /* all FSX_CROSS_PB records of a batch are valid: past the
+             * z's last pose they repeat it, so the pose loop reads them
+             * without a clamp */
+            if (tid < FSX_CROSS_PB) sP[tid] = rpf;
             /* staging: four loads per thread in flight (an index past the end
              * reloads element b0, whose store is then skipped), then the stores */
             if (!staged) {
@@ -983,8 +988,8 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
              * staged with the ligand rows */
             for (int pb = 0; pb < cnt; pb += FSX_CROSS_PB) {
             if (pb > 0) {
-                if (tid < FSX_CROSS_PB && pb + tid < cnt) {
-                    const int4 r4 = recs[start + pb + tid];
+                if (tid < FSX_CROSS_PB) {
+                    const int4 r4 = recs[start + min(pb + tid, cnt - 1)];
                     sP[tid] = make_int2(r4.y, (r4.z << 3) | (r4.w - g20));
                 }
                 lds_barrier();
@@ -1005,7 +1010,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
                     {
                         E rv = rr[0][j], rd = rr[1][j], rw = rr[2][j];
                         E uv = sUp[ec], ud = sUp[nc + ec], uw = sUp[2 * nc + ec];
-                        const F w_r = ok ? wr[j] : (F)0, w_i = ok ? wi[j] : (F)0;
+                        const F w_r = wr[j], w_i = wi[j];   /* zero past the last slot */
                         /* twiddled receptor factor: w * R^X */
                         F vr = w_r * rv.re - w_i * rv.im, vi = w_r * rv.im + w_i * rv.re;
                         F dr = w_r * rd.re - w_i * rd.im, di = w_r * rd.im + w_i * rd.re;
@@ -1070,7 +1075,7 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
              * last pose repeats it and does not write) */
             for (int pp = wave; pb + PW * pp < pe; pp += nwave) {
                 const int p = pb + PW * pp + half;
-                load_pose(min(p, pe - 1), pb, rrc, wrc, wic, prowc);
+                load_pose(p, pb, rrc, wrc, wic, prowc);
                 /* every load of the pose is issued before the first use */
                 __builtin_amdgcn_sched_barrier(0);
                 pair_pose(rrc, wrc, wic, prowc, p);
@@ -2734,7 +2739,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     p->nent = 8 * (size_t)K2 + 2;
     p->ent = (int *)malloc(sizeof(int) * p->nent);
     p->coff = (long *)malloc(sizeof(long) * (N + 1));
-    p->tw2 = (double *)malloc(sizeof(double) * 2 * (size_t)na * N);
+    p->tw2 = (double *)calloc(2 * (size_t)na * (N + 1), sizeof(double));
     if (!p->ent || !p->coff || !p->tw2) {
         plan_free_host(p);
         return FSX_ERR_NOMEM;
@@ -2765,7 +2770,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     /* the cross-term kernel addresses a q's receptor rows and a row of the
      * twiddle table by 32-bit byte offsets from 24-bit products */
     if ((size_t)p->nrows * 3 * ncmax * p->es >= ((size_t)1 << 32) || p->nrows >= (1 << 24) ||
-        na >= (1 << 24) || (size_t)na * (2 * L + 1) * p->es >= ((size_t)1 << 32)) {
+        na >= (1 << 24) || (size_t)na * (2 * L + 2) * p->es >= ((size_t)1 << 32)) {
         plan_free_host(p);
         return FSX_ERR_ARG;
     }
@@ -2794,13 +2799,13 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
             /* half set: the m > 0 terms stand for m and -m (an exact
              * factor 2) */
             const double f = p->half && m > 0 ? 2.0 : 1.0;
-            p->tw2[2 * ((long)a2 * N + m + L)] = f * A->tw_a[2 * k];
-            p->tw2[2 * ((long)a2 * N + m + L) + 1] = f * A->tw_a[2 * k + 1];
+            p->tw2[2 * ((long)a2 * (N + 1) + m + L)] = f * A->tw_a[2 * k];
+            p->tw2[2 * ((long)a2 * (N + 1) + m + L) + 1] = f * A->tw_a[2 * k + 1];
         }
     /* the mixed form reads them as FP32 pairs, in place */
     if (p->mixed) {
         float *f32 = (float *)p->tw2;
-        for (long i = 0; i < 2L * na * N; i++) f32[i] = (float)p->tw2[i];
+        for (long i = 0; i < 2L * na * (N + 1); i++) f32[i] = (float)p->tw2[i];
     }
     const int *csz = ent + 4 * K2;
     p->coff[0] = 0;
@@ -2835,7 +2840,7 @@ static int plan_init(fsx_plan *p, const fsx_hip_args *A, int tile_rows, int slot
     z[P_ACC] = sizeof(double) * 6 * Q * (size_t)n;
     z[P_FITSC] = sizeof(double) * (size_t)(n > 0 ? n : 1);
     z[P_ENT] = sizeof(int) * p->nent;
-    z[P_TW2] = sizeof(double) * 2 * (size_t)na * N;
+    z[P_TW2] = sizeof(double) * 2 * (size_t)na * (N + 1);   /* rows of 2L + 2 entries */
     z[P_COFF] = sizeof(long) * (C + 1);
     z[P_NEXT] = sizeof(int);
     z[P_TALLY] = FSX_TALLY_N * sizeof(unsigned long long);
