@@ -721,9 +721,9 @@ cross_body(double *acc, const E *Rc, const E *Bc, const decltype(E::re) *T, cons
      * sl, sl + LP, ... (J per lane) */
     constexpr int LP = FSX_WAVE / PW;
     const int sl = lane % LP, half = lane / LP;
-    /* bytes per receptor row (three parts of nc entries), the lane's slot
-     * within a row, and the step to its last slot (back to slot 0 when that
-     * one is past the last slot) */
+    /* bytes per receptor row and per row of U in LDS (three parts of nc
+     * entries each), the lane's slot within a row, and the step to its last
+     * slot (back to slot 0 when that one is past the last slot) */
     const unsigned rstride = 3u * (unsigned)nc * (unsigned)sizeof(E);
     const unsigned soff = (unsigned)(sl * sizeof(E));
     const unsigned dlast = sl + LP * (J - 1) < ns ? (unsigned)(LP * (J - 1) * sizeof(E))
