@@ -1683,18 +1683,21 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
  * meet by an xor butterfly, after which every lane of the group holds the
  * same value and gradient and runs the same solver step.  Between steps the
  * solver state lives in LDS, one copy per group (every lane of the group
- * writes the same values), so that the kernel fits in 168 registers: three
- * wavefronts per SIMD, against one for the compact form's state.  With four
- * lanes per fit (Q up to 64) a wavefront runs 16 fits, so the solver's
- * divergent code is shared by many fits while the evaluation of each is
- * split four ways (on the 210,000-pose L = 15 workload: 1.64 ms, against
- * 1.83 ms at two wavefronts per SIMD and 1.87 ms with eight lanes).  The moments, q and the step reciprocals sit in LDS, slot
+ * writes the same values), so that the kernel fits in 128 registers with
+ * 144 bytes of spills: four wavefronts per SIMD, against one for the
+ * compact form's state (on the 210,000-pose L = 15 workload 1.50 ms,
+ * against 1.60 ms at three wavefronts in 168 registers without spills).
+ * With four lanes per fit (Q up to 64) a wavefront runs 16 fits, so the
+ * solver's divergent code is shared by many fits while the evaluation of
+ * each is split four ways (at three wavefronts per SIMD: 1.64 ms, against
+ * 1.83 ms with two lanes and 1.87 ms with eight).  The moments, q and the
+ * step reciprocals sit in LDS, slot
  * ii * fpl + j for q = j qpl + ii, so the lanes of a group read consecutive
  * doubles.  Groups take poses from a queue (next) until none is left. */
 #define FSX_FG_ARR 8     /* LDS arrays: a0..a5, q, 1 / (q_i - q_{i-1}) */
 #define FSX_FG_QLANE 16  /* q per lane the default lanes per fit aim at */
 #ifndef FSX_FG_WAVES
-#define FSX_FG_WAVES 3   /* wavefronts per SIMD the register budget allows */
+#define FSX_FG_WAVES 4   /* wavefronts per SIMD (the register budget of the launch bounds) */
 #endif
 
 __device__ __forceinline__ void fitg_eval(const float *pr, const double *sNt, int ns, int qpl,
