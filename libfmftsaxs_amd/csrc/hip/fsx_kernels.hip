@@ -1684,7 +1684,8 @@ k_fit(const double *acc, const int4 *recs, const int *nvalid_p, int Q, const dou
  * same value and gradient and runs the same solver step.  Between steps the
  * solver state lives in LDS, one copy per group (every lane of the group
  * writes the same values), so that the kernel fits in 128 registers with
- * 144 bytes of spills: four wavefronts per SIMD, against one for the
+ * 144 bytes of spills (reloaded once per evaluation or per fit, none in the
+ * loop over q): four wavefronts per SIMD, against one for the
  * compact form's state (on the 210,000-pose L = 15 workload 1.50 ms,
  * against 1.60 ms at three wavefronts in 168 registers without spills).
  * With four lanes per fit (Q up to 64) a wavefront runs 16 fits, so the
